@@ -1,0 +1,173 @@
+/*
+ * pycd_hip.h -- C ABI of libpycd_hip.so, the MI355X (gfx950) implementation
+ * of PyCD's kinetic Monte Carlo hot path.
+ *
+ * PyCD itself is pure Python and has no FFI: its hot path is three methods
+ * of its Python classes.  Each entry point below replaces the numerical core
+ * of one of them and is bound from Python with ctypes (pycd_amd/_native.py):
+ *
+ *   pycd_pair_table       <- System.get_precomputed_array
+ *                            (PyCD/core.py:1611-1679; kernels for
+ *                            pot_r_ewald :799-814, pot_k_ewald :853-878)
+ *   pycd_kmc_*            <- Run.do_kmc_steps loop body
+ *                            (PyCD/core.py:2787-2861 with get_process_rates
+ *                            :1989-2050 and get_process_attributes :1946-1987)
+ *   pycd_msd              <- Analysis.compute_msd lag loop
+ *                            (PyCD/core.py:2996-3007)
+ *   pycd_kmc_batched_*    <- many independent Run.do_kmc_steps trajectories
+ *                            (PyCD/core.py:2668 loop over trajectories) in one
+ *                            persistent launch
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Every buffer is owned by the caller and
+ *     may be host or device memory unless the comment says otherwise; the
+ *     library detects which with hipPointerGetAttributes.
+ *   - Return value: 0 = done, >0 = resumable condition (see PYCD_NEED_*),
+ *     <0 = error; the message is then available from pycd_last_error().
+ *   - A context owns one HIP stream on one device and must be used from one
+ *     host thread at a time.  No C++ exception crosses this ABI.
+ *   - All physical quantities are in atomic units (bohr, Hartree, au time)
+ *     exactly as the reference holds them after its unit conversions.
+ */
+#ifndef PYCD_HIP_H
+#define PYCD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYCD_OK 0
+#define PYCD_NEED_UNIFORMS 1   /* uniform buffer consumed before the end */
+#define PYCD_ERR_ARG -1
+#define PYCD_ERR_HIP -2
+#define PYCD_ERR_NOMEM -3
+#define PYCD_ERR_STATE -4
+
+#define PYCD_PREC_F64 0     /* reciprocal sum in FP64 (drop-in)            */
+#define PYCD_PREC_F32 1     /* reciprocal sum on FP32 MFMA                 */
+#define PYCD_PREC_BF16X3 2  /* three BF16 MFMA products, FP32 accumulation */
+
+typedef struct pycd_ctx pycd_ctx;
+
+/* Context: device selection, stream, scratch.  (No reference counterpart:
+ * PyCD has no device state.) */
+int pycd_ctx_create(int device, pycd_ctx** out);
+void pycd_ctx_destroy(pycd_ctx* ctx);
+const char* pycd_last_error(const pycd_ctx* ctx);
+/* Symbol present so that loaders can check the ABI revision. */
+int pycd_abi_version(void);
+
+/* ---------------------------------------------------------------- Ewald -- */
+typedef struct {
+  int64_t n_kvec;        /* half-space k vectors with |k|^2 < k_cut^2       */
+  int64_t n_real_pairs;  /* ordered pairs (incl. diagonal) with r < r_cut   */
+  int32_t k_max[3];      /* ceil(k_cut / |b_i|)                             */
+  int32_t precision;     /* PYCD_PREC_*                                     */
+  double volume;         /* |T0 . (T1 x T2)|                                */
+  double ms_sf;          /* structure-factor kernel time                    */
+  double ms_real;        /* fused real-space share (estimated, see DESIGN)  */
+  double ms_recip;       /* reciprocal contraction + epilogue kernel time   */
+  double ms_total;       /* device time of the whole table                  */
+} pycd_ewald_report;
+
+/* Ewald pair table A (N x N, row-major FP64) with E = q^T A q:
+ *   A = A_real/eps + A_recip/eps - sqrt(alpha/pi)/eps I
+ * following System.get_precomputed_array (PyCD/core.py:1594-1661).
+ *   pos        N x 3 site coordinates (bohr)
+ *   cell       3x3 row-major minimum-image cell, rows = system_size[i] *
+ *              lattice[i] (PyCD/core.py:296)
+ *   trans      3x3 row-major translational matrix used for the volume and
+ *              the reciprocal lattice (np.multiply(system_size, lattice),
+ *              PyCD/core.py:709-710)
+ *   pbc        periodic flags per axis
+ *   alpha      the reference's alpha after its unit conversion (used as
+ *              eta^2: erfc(sqrt(alpha) r), exp(-k^2/(4 alpha)))
+ *   r_cut, k_cut, eps_r   as in the reference, in bohr / bohr^-1
+ *   precision  PYCD_PREC_*
+ *   table_out  N*N doubles, host or device
+ */
+int pycd_pair_table(pycd_ctx* ctx, const double* pos, int64_t n,
+                    const double cell[9], const double trans[9],
+                    const int32_t pbc[3], double alpha, double r_cut,
+                    double k_cut, double eps_r, int32_t precision,
+                    double* table_out, pycd_ewald_report* report);
+
+/* ------------------------------------------------------------- plain KMC -- */
+/* Inputs of one trajectory family (one table, one set of hop tables).
+ * Per-site hop slots follow the process order of get_process_attributes
+ * (PyCD/core.py:1957-1978): hop-distance type, then ascending neighbour.
+ * Every carrier's site has exactly `z` slots (uniform neighbour counts). */
+typedef struct {
+  int64_t n_sites;            /* N                                          */
+  int32_t n_carriers;         /* n_c                                        */
+  int32_t z;                  /* hop slots per carrier site                 */
+  int32_t max_z;              /* row stride of the slot tables              */
+  int32_t pad0;
+  const double* table;        /* N x N pair table (host or device)          */
+  const double* lattice_q;    /* N lattice charges (no carriers)            */
+  const int32_t* nbr_site;    /* N x max_z target site per slot             */
+  const double* nbr_vec;      /* N x max_z x 3 hop vector per slot (bohr)   */
+  const double* nbr_lambda;   /* N x max_z reorganisation energy            */
+  const double* nbr_vab;      /* N x max_z electronic coupling              */
+  const double* site_energy;  /* N relative site energies                   */
+  const double* carrier_q;    /* n_c carrier charges                        */
+  double field[3];            /* electric field (au, no conversion)         */
+  double nu;                  /* attempt frequency (1 / au time)            */
+  double kT;                  /* temperature (Hartree)                      */
+  double time_interval;       /* path-step width (au time)                  */
+  int64_t n_path;             /* P = int(t_final / time_interval) + 1       */
+} pycd_kmc_problem;
+
+typedef struct pycd_kmc pycd_kmc;
+
+/* Uploads the tables, forms phi_lattice = A q_lattice and the self terms
+ * A[s,s] - A[s,n] of every slot.  The table pointer may be released by the
+ * caller afterwards. */
+int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
+                    pycd_kmc** out);
+void pycd_kmc_destroy(pycd_kmc* s);
+
+/* Starts a trajectory: carriers at `occupancy` (n_c sites), clock 0,
+ * energy = ewald_neut + q^T A q (PyCD/core.py:2773-2780). */
+int pycd_kmc_begin(pycd_kmc* s, const int64_t* occupancy, double ewald_neut);
+
+/* Advances the trajectory, consuming two uniforms per hop (u1 selects the
+ * process, u2 advances the clock, PyCD/core.py:2799-2803) from `uniforms`
+ * (2*n_pairs doubles, host or device), until the path is full, `max_hops`
+ * hops have been made in total (max_hops <= 0: no limit), or the buffer is
+ * used up.  Per hop it writes, if the pointers are non-null, the clock after
+ * the hop to hop_time[k] and the chosen process index to hop_proc[k]
+ * (k = hop index within this call; host or device, n_pairs capacity).
+ * Returns PYCD_OK when finished, PYCD_NEED_UNIFORMS when the buffer ran out
+ * first.  *hops_out = hops made by this call. */
+int pycd_kmc_advance(pycd_kmc* s, const double* uniforms, int64_t n_pairs,
+                     int64_t max_hops, double* hop_time, int32_t* hop_proc,
+                     int64_t* hops_out);
+
+/* Copies out the trajectory outputs (any pointer may be null):
+ *   unwrapped  P x 3 n_c   (unwrapped_traj.npy)
+ *   energy     P           (energy_traj.npy)
+ *   delg_0     P           (delG0_traj.npy)
+ *   drift      n_c x 3     sum of hop vector * rate (PyCD/core.py:2822-2825)
+ *   occupancy  n_c         current sites
+ *   scalars    [t, energy, start_path_index, hops]                          */
+int pycd_kmc_read(pycd_kmc* s, double* unwrapped, double* energy,
+                  double* delg_0, double* drift, int64_t* occupancy,
+                  double scalars[4]);
+
+/* ------------------------------------------------------------------ MSD -- */
+/* Squared displacement per lag (PyCD/core.py:2996-3007):
+ *   sd[t, tau, c] = mean_{t0 < P - tau} |pos[t, t0 + tau, c] - pos[t, t0, c]|^2
+ * for tau in [1, n_lags), sd[t, 0, c] = 0.
+ *   pos  n_traj x P x n_c x 3 (already in output units), host or device
+ *   sd   n_traj x n_lags x n_c                                              */
+int pycd_msd(pycd_ctx* ctx, const double* pos, int64_t n_traj,
+             int64_t n_path, int64_t n_carriers, int64_t n_lags, double* sd);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYCD_HIP_H */

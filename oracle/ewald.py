@@ -1,0 +1,183 @@
+"""Oracle: Ewald pair table of PyCD's `System` (test infrastructure only).
+
+Restates, with the reference's operation order:
+  * minimum image   PyCD/core.py:292-361 (`_setup_efficient_pbc`,
+                    `apply_minimum_image_convention`)
+  * geometry        PyCD/core.py:708-726 (translational matrix, volume,
+                    reciprocal lattice) and :768-784 (unit handling of
+                    alpha, r_cut, k_cut)
+  * real space      PyCD/core.py:799-814 (`pot_r_ewald`)
+  * k half-space    PyCD/core.py:816-838 (`get_effective_k_vectors`)
+  * reciprocal      PyCD/core.py:853-878 (`pot_k_ewald`) with k_max from
+                    PyCD/core.py:1598-1602
+  * assembly        PyCD/core.py:1594-1596, 1656-1661
+"""
+import numpy as np
+from scipy.special import erfc
+
+from pycd_amd import constants
+
+
+def simulation_cell(lattice_bohr, system_size):
+    """Row-scaled cell used by the minimum image (core.py:296)."""
+    return lattice_bohr * np.asarray(system_size)[:, np.newaxis]
+
+
+def mic_rows(d, cell, pbc):
+    """Minimum image of each row of `d` (core.py:304-361).
+
+    Base offset -round(frac) on periodic axes, then the 26 neighbouring
+    offsets in (dx, dy, dz) loop order; a candidate replaces the best one
+    only when its norm is strictly smaller.
+    """
+    d = np.atleast_2d(np.asarray(d, dtype=np.float64))
+    inv = np.linalg.inv(cell)
+    mask = np.asarray(pbc, dtype=bool)
+    frac = d @ inv
+    base = np.zeros_like(frac)
+    base[:, mask] = -np.round(frac[:, mask])
+    best = (frac + base) @ cell
+    best_n = np.sqrt(np.einsum("ij,ij->i", best, best))
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                if dx == 0 and dy == 0 and dz == 0:
+                    continue
+                delta = np.array([dx if mask[0] else 0, dy if mask[1] else 0,
+                                  dz if mask[2] else 0], dtype=np.float64)
+                if not np.any(delta != 0):
+                    continue
+                cand = (frac + (base + delta)) @ cell
+                cand_n = np.sqrt(np.einsum("ij,ij->i", cand, cand))
+                better = cand_n < best_n
+                best[better] = cand[better]
+                best_n[better] = cand_n[better]
+    return best
+
+
+def pairwise_vectors(pos, cell, pbc):
+    """N x N x 3 minimum-image vectors pos[j] - pos[i] (core.py:571-594)."""
+    n = len(pos)
+    out = np.empty((n, n, 3))
+    for i in range(n):
+        out[i] = mic_rows(pos - pos[i], cell, pbc)
+    return out
+
+
+def geometry(lattice_bohr, system_size):
+    """Translational matrix, volume and reciprocal lattice (core.py:708-726).
+
+    The translational matrix is np.multiply(system_size, lattice): each
+    COLUMN j is scaled by system_size[j] (reference quirk; equal to the row
+    scaling of `simulation_cell` for n x n x n supercells of cells whose
+    off-diagonal entries pair equal sizes).
+    """
+    t = np.multiply(np.asarray(system_size), lattice_bohr)
+    volume = abs(np.dot(t[0], np.cross(t[1], t[2])))
+    recip = (2 * np.pi / volume
+             * np.array([np.cross(t[1], t[2]), np.cross(t[2], t[0]),
+                         np.cross(t[0], t[1])]))
+    return t, volume, recip
+
+
+def ewald_units(alpha_ang, r_cut_ang, k_cut_ang):
+    """YAML units -> what the reference uses (core.py:768-784).
+
+    alpha is divided by ANG2BOHR and then used as eta^2 everywhere.
+    """
+    return (alpha_ang / constants.ANG2BOHR, r_cut_ang * constants.ANG2BOHR,
+            k_cut_ang / constants.ANG2BOHR)
+
+
+def pot_r(pairwise, alpha, r_cut):
+    """Real-space term inside the simulation cell (core.py:799-814)."""
+    n = pairwise.shape[0]
+    out = np.zeros((n, n))
+    r = np.linalg.norm(pairwise, axis=2)
+    inside = r < r_cut
+    out[inside] += erfc(np.sqrt(alpha) * r[inside]) / 2
+    npairs = int(inside.sum())
+    np.fill_diagonal(r, 1)
+    out[inside] /= r[inside]
+    return out, npairs
+
+
+def half_space_kvectors(k_max):
+    """Integer k triples, one of each +/- pair, in mgrid order (:816-838)."""
+    grid = np.mgrid[-k_max[0]:k_max[0] + 1, -k_max[1]:k_max[1] + 1,
+                    -k_max[2]:k_max[2] + 1]
+    allv = grid.reshape(3, -1).T
+    allv = allv[~np.all(allv == 0, axis=1)]
+    kept, seen = [], set()
+    for v in allv:
+        t = tuple(int(x) for x in v)
+        if t not in seen:
+            kept.append(v)
+            seen.add(tuple(-x for x in t))
+    return np.array(kept)
+
+
+def k_max_for(k_cut, recip):
+    """k_max = ceil(k_cut / |b_i|) (core.py:1599)."""
+    return np.ceil(k_cut / np.linalg.norm(recip, axis=1)).astype(int)
+
+
+def pot_k(pairwise, recip, volume, k_max, alpha, k_cut):
+    """Reciprocal-space term (core.py:853-878): loop over the half space."""
+    n = pairwise.shape[0]
+    out = np.zeros((n, n))
+    alpha4 = 4 * alpha
+    coeff = (2 * np.pi) / volume
+    k_cut_2 = k_cut ** 2
+    for kv in half_space_kvectors(k_max):
+        k = np.dot(kv, recip)
+        k2 = np.dot(k, k)
+        if k2 < k_cut_2:
+            out += (coeff * np.exp(-k2 / alpha4)
+                    * np.cos(np.tensordot(pairwise, k, axes=([2], [0]))) / k2)
+    out *= 2
+    return out
+
+
+def pair_table(pairwise, lattice_bohr, system_size, alpha, r_cut, k_cut,
+               eps_r):
+    """A = A_r/eps + A_k/eps - sqrt(alpha/pi)/eps * I (core.py:1594-1661)."""
+    _, volume, recip = geometry(lattice_bohr, system_size)
+    real = pot_r(pairwise, alpha, r_cut)[0] / eps_r
+    km = k_max_for(k_cut, recip)
+    fourier = pot_k(pairwise, recip, volume, km, alpha, k_cut) / eps_r
+    self_term = -np.eye(pairwise.shape[0]) * np.sqrt(alpha / np.pi) / eps_r
+    return real + fourier + self_term
+
+
+def table_rows(pos, rows, lattice_bohr, system_size, pbc, alpha, r_cut,
+               k_cut, eps_r):
+    """Selected rows of `pair_table`, from positions, for large N.
+
+    Same formulas as pot_r / pot_k / the assembly above, evaluated only for
+    the requested centre rows (minimum-image vectors pos[j] - pos[i]).
+    """
+    cell = simulation_cell(lattice_bohr, system_size)
+    _, volume, recip = geometry(lattice_bohr, system_size)
+    km = k_max_for(k_cut, recip)
+    kv = half_space_kvectors(km) @ recip
+    k2 = np.einsum("ij,ij->i", kv, kv)
+    keep = k2 < k_cut ** 2
+    kv, k2 = kv[keep], k2[keep]
+    coeff = (2 * np.pi) / volume
+    w = coeff * np.exp(-k2 / (4 * alpha)) / k2
+    out = np.zeros((len(rows), len(pos)))
+    for r_i, i in enumerate(rows):
+        v = mic_rows(pos - pos[i], cell, pbc)
+        r = np.linalg.norm(v, axis=1)
+        real = np.zeros(len(pos))
+        inside = r < r_cut
+        real[inside] = erfc(np.sqrt(alpha) * r[inside]) / 2
+        r1 = r.copy()
+        r1[i] = 1.0
+        real[inside] /= r1[inside]
+        fourier = 2 * (np.cos(v @ kv.T) @ w)
+        row = real / eps_r + fourier / eps_r
+        row[i] += -np.sqrt(alpha / np.pi) / eps_r
+        out[r_i] = row
+    return out

@@ -1,0 +1,140 @@
+"""Oracle: one trajectory of `Run.do_kmc_steps` in NumPy (test infrastructure).
+
+Restates PyCD/core.py:1946-2050 (`get_process_attributes`,
+`get_process_rates`) and the loop body PyCD/core.py:2787-2861 with the
+reference's own NumPy operations and order:
+
+  term01 = dot(q, A[n,:] - A[s,:]);  term02 = q_s (A[s,s] - A[s,n])
+  dG0    = 2 q_s (term01 + term02) + (E[n] - E[s])
+  dGs    = ((lam + dG0)^2 / (4 lam) - V_ab) - 0.5 F.d
+  k      = nu * e**(-dGs / kT)
+  k_tot  = builtin sum(k);  cum = cumsum(k / k_tot);  p = first cum > u1
+  t     -= log(u2) / k_tot;  end = int(t / dt)
+
+followed by the occupancy/charge update and the path-step binning rule.
+
+Input tables are per site: `nbr[s, j]`, `vec[s, j]`, `lam[s, j]`,
+`vab[s, j]` for the j-th hop slot of site s (slots ordered by hop-distance
+type, then by ascending neighbour index, PyCD/core.py:1957-1978), with
+`count[s]` valid slots.
+"""
+import numpy as np
+
+
+class Problem:
+    """Plain container of one trajectory's inputs (all atomic units)."""
+
+    def __init__(self, table, lattice_q, count, nbr, vec, lam, vab,
+                 site_energy, carrier_q, field, nu, kT, time_interval,
+                 n_path):
+        self.table = table
+        self.lattice_q = np.asarray(lattice_q, dtype=np.float64)
+        self.count = np.asarray(count)
+        self.nbr = np.asarray(nbr)
+        self.vec = np.asarray(vec, dtype=np.float64)
+        self.lam = np.asarray(lam, dtype=np.float64)
+        self.vab = np.asarray(vab, dtype=np.float64)
+        self.site_energy = np.asarray(site_energy, dtype=np.float64)
+        self.carrier_q = np.asarray(carrier_q, dtype=np.float64)
+        self.field = np.asarray(field, dtype=np.float64)
+        self.nu = float(nu)
+        self.kT = float(kT)
+        self.time_interval = float(time_interval)
+        self.n_path = int(n_path)
+
+
+def run(pb, occupancy, uniform, max_hops=None, record=False,
+        ewald_neut=0.0):
+    """Run until the path is full (or `max_hops` hops).
+
+    `uniform()` returns the next U[0,1) draw (two per hop: selection, clock).
+    Returns a dict with time_data, unwrapped (P x 3 n_c), energy and delg_0
+    path arrays, drift accumulator, final occupancy and, if `record`, the
+    chosen process index of every hop.
+    """
+    A = pb.table
+    occ = [int(s) for s in occupancy]
+    nc = len(occ)
+    q = pb.lattice_q.copy()
+    for c, s in enumerate(occ):
+        q[s] += pb.carrier_q[c]
+    P = pb.n_path
+    field_on = bool(np.any(pb.field != 0))
+    unwrapped = np.zeros((P, 3 * nc))
+    energy = np.zeros(P)
+    delg0_path = np.zeros(P)
+    disp = np.zeros(3 * nc)
+    drift = np.zeros((nc, 3))
+    energy_now = ewald_neut + np.sum(np.multiply(np.outer(q, q), A))
+    energy[0] = energy_now
+    time_data = [0.0]
+    procs = []
+    start = end = 1
+    t = 0.0
+    hops = 0
+    while end < P:
+        if max_hops is not None and hops >= max_hops:
+            break
+        olds, news, owners, slots = [], [], [], []
+        for c, s in enumerate(occ):
+            for j in range(int(pb.count[s])):
+                olds.append(s)
+                news.append(int(pb.nbr[s, j]))
+                owners.append(c)
+                slots.append(j)
+        n_proc = len(olds)
+        k_list = np.zeros(n_proc)
+        dg0 = np.zeros(n_proc)
+        for p in range(n_proc):
+            s, n, c, j = olds[p], news[p], owners[p], slots[p]
+            qs = pb.carrier_q[c]
+            term01 = np.dot(q, A[n, :] - A[s, :])
+            term02 = qs * (A[s, s] - A[s, n])
+            ewald = 2 * qs * (term01 + term02)
+            d = ewald + (pb.site_energy[n] - pb.site_energy[s])
+            dg0[p] = d
+            lam = pb.lam[s, j]
+            shift = 0.5 * np.dot(pb.field, pb.vec[s, j]) if field_on else 0
+            dgs = (((lam + d) ** 2 / (4 * lam)) - pb.vab[s, j]) - shift
+            k_list[p] = pb.nu * np.e ** (-dgs / pb.kT)
+        k_total = sum(k_list)
+        cum = (k_list / k_total).cumsum()
+        u1 = uniform()
+        p = int(np.where(cum > u1)[0][0])
+        u2 = uniform()
+        t -= np.log(u2) / k_total
+        end = int(t / pb.time_interval)
+        delg0_path[start:end] = dg0[p]
+        c = owners[p]
+        s_old, s_new = olds[p], news[p]
+        occ[c] = s_new
+        disp[3 * c:3 * c + 3] += pb.vec[s_old, slots[p]]
+        if field_on:
+            drift[c] += pb.vec[s_old, slots[p]] * k_list[p]
+        energy_now += dg0[p]
+        q[s_old] -= pb.carrier_q[c]
+        q[s_new] += pb.carrier_q[c]
+        time_data.append(t)
+        if record:
+            procs.append(p)
+        hops += 1
+        if end >= start + 1:
+            if end >= P:
+                end = P
+            unwrapped[start:end] = unwrapped[start - 1] + disp
+            energy[start:end] = energy_now
+            disp = np.zeros(3 * nc)
+            start = end
+    return {
+        "time_data": np.asarray(time_data),
+        "unwrapped": unwrapped,
+        "energy": energy,
+        "delg_0": delg0_path,
+        "drift": drift,
+        "occupancy": occ,
+        "procs": np.asarray(procs, dtype=np.int64),
+        "hops": hops,
+        "start": start,
+        "disp": disp,
+        "energy_now": energy_now,
+    }

@@ -1,0 +1,299 @@
+"""ctypes binding of libpycd_hip.so (declared in include/pycd_hip.h).
+
+The library is required: if it is missing or fails to load, every entry
+point raises.  There is no CPU fallback on this path.
+"""
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libpycd_hip.so"
+
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+OK, NEED_UNIFORMS = 0, 1
+
+_lib = None
+_ctx = {}
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+class EwaldReport(C.Structure):
+    _fields_ = [("n_kvec", C.c_int64), ("n_real_pairs", C.c_int64),
+                ("k_max", C.c_int32 * 3), ("precision", C.c_int32),
+                ("volume", C.c_double), ("ms_sf", C.c_double),
+                ("ms_real", C.c_double), ("ms_recip", C.c_double),
+                ("ms_total", C.c_double)]
+
+
+class KmcProblemC(C.Structure):
+    _fields_ = [("n_sites", C.c_int64), ("n_carriers", C.c_int32),
+                ("z", C.c_int32), ("max_z", C.c_int32), ("pad0", C.c_int32),
+                ("table", C.c_void_p), ("lattice_q", C.c_void_p),
+                ("nbr_site", C.c_void_p), ("nbr_vec", C.c_void_p),
+                ("nbr_lambda", C.c_void_p), ("nbr_vab", C.c_void_p),
+                ("site_energy", C.c_void_p), ("carrier_q", C.c_void_p),
+                ("field", C.c_double * 3), ("nu", C.c_double),
+                ("kT", C.c_double), ("time_interval", C.c_double),
+                ("n_path", C.c_int64)]
+
+
+_P = C.c_void_p
+_SIGNATURES = {
+    "pycd_abi_version": (C.c_int, []),
+    "pycd_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "pycd_ctx_destroy": (None, [_P]),
+    "pycd_last_error": (C.c_char_p, [_P]),
+    "pycd_pair_table": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P, C.c_double,
+                                  C.c_double, C.c_double, C.c_double,
+                                  C.c_int32, _P, C.POINTER(EwaldReport)]),
+    "pycd_kmc_create": (C.c_int, [_P, C.POINTER(KmcProblemC),
+                                  C.POINTER(C.c_void_p)]),
+    "pycd_kmc_destroy": (None, [_P]),
+    "pycd_kmc_begin": (C.c_int, [_P, _P, C.c_double]),
+    "pycd_kmc_advance": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P,
+                                   C.POINTER(C.c_int64)]),
+    "pycd_kmc_read": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "pycd_msd": (C.c_int, [_P, _P, C.c_int64, C.c_int64, C.c_int64,
+                           C.c_int64, _P]),
+}
+
+
+def exported_symbols():
+    """Names the binding expects the library to export."""
+    return sorted(_SIGNATURES)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise NativeError(
+                f"{LIB_PATH} is missing; build it with "
+                f"`python -m pycd_amd.build` (there is no CPU fallback)")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def _check(ctx, rc, what):
+    if rc < 0:
+        msg = lib().pycd_last_error(ctx)
+        raise NativeError(f"{what} failed ({rc}): "
+                          f"{msg.decode() if msg else '?'}")
+    return rc
+
+
+def context(device=None):
+    """The process-wide context of `device` (default: PYCD_DEVICE or 0)."""
+    if device is None:
+        device = int(os.environ.get("PYCD_DEVICE", "0"))
+    if device not in _ctx:
+        h = C.c_void_p()
+        rc = lib().pycd_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise NativeError(f"pycd_ctx_create({device}) failed ({rc}): no "
+                              f"usable HIP device")
+        _ctx[device] = h
+    return _ctx[device]
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+# ------------------------------------------------------------------ Ewald --
+def pair_table(pos, cell, trans, pbc, alpha, r_cut, k_cut, eps_r,
+               precision="f64", out=None):
+    """N x N Ewald table on the GPU; returns (table, report dict)."""
+    pos = _f64(pos)
+    n = pos.shape[0]
+    cell = _f64(cell, (9,))
+    trans = _f64(trans, (9,))
+    pbc = np.ascontiguousarray(pbc, dtype=np.int32).reshape(3)
+    if out is None:
+        out = np.empty((n, n), dtype=np.float64)
+    rep = EwaldReport()
+    ctx = context()
+    rc = lib().pycd_pair_table(ctx, _ptr(pos), n, _ptr(cell), _ptr(trans),
+                               _ptr(pbc), float(alpha), float(r_cut),
+                               float(k_cut), float(eps_r), PREC[precision],
+                               _ptr(out), C.byref(rep))
+    _check(ctx, rc, "pycd_pair_table")
+    report = {"n_kvec": rep.n_kvec, "n_real_pairs": rep.n_real_pairs,
+              "k_max": list(rep.k_max), "precision": precision,
+              "volume": rep.volume, "ms_sf": rep.ms_sf,
+              "ms_real": rep.ms_real, "ms_recip": rep.ms_recip,
+              "ms_total": rep.ms_total}
+    return out, report
+
+
+def table_energy_parts(report, q):
+    raise NotImplementedError(
+        "compute_energy_contributions is not part of this implementation")
+
+
+# -------------------------------------------------------------------- KMC --
+class KmcProblem:
+    """Host arrays of one trajectory family (atomic units)."""
+
+    def __init__(self, table, lattice_q, nbr_count, nbr_site, nbr_vec,
+                 nbr_lambda, nbr_vab, site_energy, carrier_q, field, nu, kT,
+                 time_interval, n_path):
+        self.table = _f64(table)
+        n = self.table.shape[0]
+        self.lattice_q = _f64(lattice_q)
+        self.nbr_count = np.ascontiguousarray(nbr_count, dtype=np.int32)
+        self.nbr_site = np.ascontiguousarray(nbr_site, dtype=np.int32)
+        self.max_z = self.nbr_site.shape[1]
+        self.nbr_vec = _f64(nbr_vec)
+        self.nbr_lambda = _f64(nbr_lambda)
+        self.nbr_vab = _f64(nbr_vab)
+        self.site_energy = _f64(site_energy)
+        self.carrier_q = _f64(carrier_q)
+        self.field = _f64(field)
+        self.nu, self.kT = float(nu), float(kT)
+        self.time_interval = float(time_interval)
+        self.n_path = int(n_path)
+        counts = set(int(c) for c in self.nbr_count if c > 0)
+        if len(counts) != 1:
+            raise ValueError("every hopping site must have the same number "
+                             f"of hop slots (found {sorted(counts)})")
+        self.z = counts.pop()
+        assert self.lattice_q.shape == (n,)
+
+
+class KmcSession:
+    """One device-resident KMC family; run() executes one trajectory."""
+
+    def __init__(self, problem, chunk=1 << 16, max_chunk=1 << 21):
+        self.pb = problem
+        self.ctx = context()
+        self.chunk0 = chunk
+        self.max_chunk = max_chunk
+        pb = problem
+        c = KmcProblemC()
+        c.n_sites = pb.table.shape[0]
+        c.n_carriers = len(pb.carrier_q)
+        c.z = pb.z
+        c.max_z = pb.max_z
+        c.table = pb.table.ctypes.data
+        c.lattice_q = pb.lattice_q.ctypes.data
+        c.nbr_site = pb.nbr_site.ctypes.data
+        c.nbr_vec = pb.nbr_vec.ctypes.data
+        c.nbr_lambda = pb.nbr_lambda.ctypes.data
+        c.nbr_vab = pb.nbr_vab.ctypes.data
+        c.site_energy = pb.site_energy.ctypes.data
+        c.carrier_q = pb.carrier_q.ctypes.data
+        for a in range(3):
+            c.field[a] = float(pb.field[a])
+        c.nu, c.kT = pb.nu, pb.kT
+        c.time_interval = pb.time_interval
+        c.n_path = pb.n_path
+        h = C.c_void_p()
+        _check(self.ctx, lib().pycd_kmc_create(self.ctx, C.byref(c),
+                                               C.byref(h)), "pycd_kmc_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().pycd_kmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def begin(self, occupancy, ewald_neut=0.0):
+        occ = np.ascontiguousarray(occupancy, dtype=np.int64)
+        _check(self.ctx, lib().pycd_kmc_begin(self.h, _ptr(occ),
+                                              float(ewald_neut)),
+               "pycd_kmc_begin")
+
+    def advance(self, uniforms, max_hops=None, record=False):
+        """Consume pairs of `uniforms`; returns (rc, times, procs)."""
+        u = _f64(uniforms)
+        n_pairs = u.size // 2
+        times = np.empty(max(n_pairs, 1), dtype=np.float64)
+        procs = np.empty(max(n_pairs, 1), dtype=np.int32) if record else None
+        made = C.c_int64(0)
+        rc = lib().pycd_kmc_advance(self.h, _ptr(u), n_pairs,
+                                    int(max_hops or 0), _ptr(times),
+                                    _ptr(procs) if record else None,
+                                    C.byref(made))
+        _check(self.ctx, rc, "pycd_kmc_advance")
+        k = made.value
+        return rc, times[:k], (procs[:k] if record else None)
+
+    def read(self):
+        pb = self.pb
+        nc = len(pb.carrier_q)
+        P = pb.n_path
+        rows = np.empty((P, 3 * nc))
+        energy = np.empty(P)
+        delg = np.empty(P)
+        drift = np.empty((nc, 3))
+        occ = np.empty(nc, dtype=np.int64)
+        sc = np.empty(4)
+        _check(self.ctx, lib().pycd_kmc_read(self.h, _ptr(rows), _ptr(energy),
+                                             _ptr(delg), _ptr(drift),
+                                             _ptr(occ), _ptr(sc)),
+               "pycd_kmc_read")
+        return {"unwrapped": rows, "energy": energy, "delg_0": delg,
+                "drift": drift, "occupancy": [int(v) for v in occ],
+                "t": sc[0], "energy_now": sc[1], "start": int(sc[2]),
+                "hops": int(sc[3])}
+
+    def run(self, occupancy, stream, ewald_neut=0.0, max_hops=None,
+            record=False):
+        """Whole trajectory from `occupancy`, uniforms from `stream.take`."""
+        self.begin(occupancy, ewald_neut)
+        chunk = self.chunk0
+        times, procs = [np.zeros(1)], []
+        while True:
+            if max_hops is not None:
+                done = sum(len(t) for t in times) - 1
+                chunk = max(1, min(chunk, int(max_hops) - done))
+            u = stream.take(2 * chunk)
+            rc, t, p = self.advance(u, max_hops=max_hops, record=record)
+            times.append(t)
+            if record:
+                procs.append(p)
+            if rc == OK:
+                break
+            chunk = min(chunk * 2, self.max_chunk)
+        out = self.read()
+        out["time_data"] = np.concatenate(times)
+        out["procs"] = (np.concatenate(procs).astype(np.int64) if record
+                        else None)
+        return out
+
+
+# -------------------------------------------------------------------- MSD --
+def msd(pos, n_lags):
+    """sd[t, tau, c] for pos of shape (n_traj, P, n_c, 3)."""
+    pos = _f64(pos)
+    n_traj, P, nc, three = pos.shape
+    assert three == 3
+    sd = np.empty((n_traj, int(n_lags), nc))
+    ctx = context()
+    _check(ctx, lib().pycd_msd(ctx, _ptr(pos), n_traj, P, nc, int(n_lags),
+                               _ptr(sd)), "pycd_msd")
+    return sd

@@ -1,0 +1,103 @@
+// Shared host-side plumbing of libpycd_hip.so: context, errors, buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pycd_hip.h"
+
+struct pycd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  int cu_count = 0;
+};
+
+namespace pycd {
+
+inline int fail(pycd_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->error = buf;
+  return code;
+}
+
+#define PYCD_HIP(ctx, expr)                                                   \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return ::pycd::fail((ctx), PYCD_ERR_HIP, "%s failed: %s (%s:%d)", #expr, \
+                          hipGetErrorString(e_), __FILE__, __LINE__);         \
+  } while (0)
+
+// True when `p` is device (or managed) memory visible to the current device.
+inline bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+// Owned device allocation.
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  hipError_t alloc(size_t n) {
+    release();
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T));
+    if (e == hipSuccess) count = n;
+    return e;
+  }
+};
+
+// Copies `n` elements from a host-or-device pointer into device memory.
+template <typename T>
+hipError_t upload(T* dst, const T* src, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipMemcpyKind kind =
+      is_device_ptr(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  return hipMemcpyAsync(dst, src, n * sizeof(T), kind, st);
+}
+
+template <typename T>
+hipError_t download(T* dst, const T* src, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipMemcpyKind kind =
+      is_device_ptr(dst) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  return hipMemcpyAsync(dst, src, n * sizeof(T), kind, st);
+}
+
+// Scoped device selection.
+struct DeviceGuard {
+  int prev = 0;
+  explicit DeviceGuard(int dev) {
+    (void)hipGetDevice(&prev);
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+}  // namespace pycd

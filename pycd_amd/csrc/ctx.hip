@@ -1,0 +1,42 @@
+// Context management of libpycd_hip.so.
+#include "common.hpp"
+
+extern "C" int pycd_abi_version(void) { return 1; }
+
+extern "C" int pycd_ctx_create(int device, pycd_ctx** out) {
+  if (!out) return PYCD_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PYCD_ERR_HIP;
+  if (device < 0 || device >= n) return PYCD_ERR_ARG;
+  pycd_ctx* ctx = new (std::nothrow) pycd_ctx();
+  if (!ctx) return PYCD_ERR_NOMEM;
+  ctx->device = device;
+  pycd::DeviceGuard g(device);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
+          hipSuccess) {
+    delete ctx;
+    return PYCD_ERR_HIP;
+  }
+  ctx->cu_count = prop.multiProcessorCount;
+  *out = ctx;
+  return PYCD_OK;
+}
+
+extern "C" void pycd_ctx_destroy(pycd_ctx* ctx) {
+  if (!ctx) return;
+  {
+    pycd::DeviceGuard g(ctx->device);
+    if (ctx->stream) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipStreamDestroy(ctx->stream);
+    }
+  }
+  delete ctx;
+}
+
+extern "C" const char* pycd_last_error(const pycd_ctx* ctx) {
+  return ctx ? ctx->error.c_str() : "null context";
+}

@@ -1,0 +1,151 @@
+// Pieces shared by the FP64 and MFMA Ewald table kernels.
+#pragma once
+
+#include "common.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace pycd {
+
+// Minimum-image geometry: frac = d S^-1, wrap periodic axes with round-half-
+// even, then try the 26 neighbouring images, keeping a strictly shorter one
+// (PyCD/core.py:304-361).  dS holds delta . S for the candidate offsets.
+struct MicGeometry {
+  double inv[9];
+  double cell[9];
+  double dS[26][3];
+  int n_delta;
+  int pbc[3];
+};
+
+struct EwaldScalars {
+  double sqrt_alpha;
+  double r_cut;
+  double eps;
+  double self_term;  // -sqrt(alpha/pi)/eps
+};
+
+struct EwaldSetup {
+  double volume = 0;
+  int k_max[3] = {0, 0, 0};
+  int K = 0;
+  std::vector<double> kv;  // K x 3 reciprocal vectors (bohr^-1)
+  std::vector<double> sw;  // sqrt of (2pi/V) exp(-k^2/(4 alpha)) / k^2
+  MicGeometry geo;
+};
+
+inline void fill_mic_geometry(const double* cell, const int32_t* pbc,
+                              MicGeometry& g) {
+  for (int k = 0; k < 9; ++k) g.cell[k] = cell[k];
+  const double* m = cell;
+  const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) -
+                     m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                     m[2] * (m[3] * m[7] - m[4] * m[6]);
+  g.inv[0] = (m[4] * m[8] - m[5] * m[7]) / det;
+  g.inv[1] = (m[2] * m[7] - m[1] * m[8]) / det;
+  g.inv[2] = (m[1] * m[5] - m[2] * m[4]) / det;
+  g.inv[3] = (m[5] * m[6] - m[3] * m[8]) / det;
+  g.inv[4] = (m[0] * m[8] - m[2] * m[6]) / det;
+  g.inv[5] = (m[2] * m[3] - m[0] * m[5]) / det;
+  g.inv[6] = (m[3] * m[7] - m[4] * m[6]) / det;
+  g.inv[7] = (m[1] * m[6] - m[0] * m[7]) / det;
+  g.inv[8] = (m[0] * m[4] - m[1] * m[3]) / det;
+  for (int a = 0; a < 3; ++a) g.pbc[a] = pbc[a] ? 1 : 0;
+  g.n_delta = 0;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int d[3] = {g.pbc[0] ? dx : 0, g.pbc[1] ? dy : 0,
+                          g.pbc[2] ? dz : 0};
+        if ((dx == 0 && dy == 0 && dz == 0) || (!d[0] && !d[1] && !d[2]))
+          continue;
+        for (int c = 0; c < 3; ++c)
+          g.dS[g.n_delta][c] = d[0] * cell[c] + d[1] * cell[3 + c] +
+                               d[2] * cell[6 + c];
+        ++g.n_delta;
+      }
+  for (int k = g.n_delta; k < 26; ++k)
+    g.dS[k][0] = g.dS[k][1] = g.dS[k][2] = 0.0;
+}
+
+__device__ inline double mic_r2(double dx, double dy, double dz,
+                                const MicGeometry& g) {
+  double f0 = dx * g.inv[0] + dy * g.inv[3] + dz * g.inv[6];
+  double f1 = dx * g.inv[1] + dy * g.inv[4] + dz * g.inv[7];
+  double f2 = dx * g.inv[2] + dy * g.inv[5] + dz * g.inv[8];
+  if (g.pbc[0]) f0 -= rint(f0);
+  if (g.pbc[1]) f1 -= rint(f1);
+  if (g.pbc[2]) f2 -= rint(f2);
+  const double bx = f0 * g.cell[0] + f1 * g.cell[3] + f2 * g.cell[6];
+  const double by = f0 * g.cell[1] + f1 * g.cell[4] + f2 * g.cell[7];
+  const double bz = f0 * g.cell[2] + f1 * g.cell[5] + f2 * g.cell[8];
+  double best = bx * bx + by * by + bz * bz;
+  for (int m = 0; m < g.n_delta; ++m) {
+    const double cx = bx + g.dS[m][0];
+    const double cy = by + g.dS[m][1];
+    const double cz = bz + g.dS[m][2];
+    const double r2 = cx * cx + cy * cy + cz * cz;
+    best = r2 < best ? r2 : best;
+  }
+  return best;
+}
+
+// Final table entry from the reciprocal accumulator `acc` (= sum over the
+// half space of w cos, i.e. (Y Y^T)[i,j]).
+__device__ inline double assemble_entry(int i, int j,
+                                        const double* __restrict__ pos,
+                                        const MicGeometry& g,
+                                        const EwaldScalars& es, double acc,
+                                        unsigned& cnt) {
+  const double dx = pos[3 * j] - pos[3 * i];
+  const double dy = pos[3 * j + 1] - pos[3 * i + 1];
+  const double dz = pos[3 * j + 2] - pos[3 * i + 2];
+  const double r = sqrt(mic_r2(dx, dy, dz, g));
+  double real = 0.0;
+  if (r < es.r_cut) {
+    ++cnt;
+    double v = erfc(es.sqrt_alpha * r) / 2;
+    if (i != j) v /= r;
+    real = v / es.eps;
+  }
+  double out = real + (2.0 * acc) / es.eps;
+  if (i == j) out += es.self_term;
+  return out;
+}
+
+__device__ inline void count_pairs(unsigned cnt,
+                                   unsigned long long* __restrict__ pairs) {
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0 && cnt)
+    atomicAdd(pairs, static_cast<unsigned long long>(cnt));
+}
+
+// Maps the 1-D workgroup index to a (row, column) tile.  Consecutive
+// workgroups are dealt round-robin to the 8 XCDs; the first step gives each
+// XCD a contiguous run of tiles, the second walks them in groups of 8 tile
+// rows so that an XCD's L2 holds the few Y row blocks it is using.
+__device__ inline void tile_coords(int n, int tile, int& bi, int& bj) {
+  const int nb = (n + tile - 1) / tile;
+  const int T = nb * nb;
+  int w = blockIdx.x;
+  const int full = (T / 8) * 8;
+  if (w < full) {
+    const int per = full / 8;
+    w = (w % 8) * per + (w / 8);
+  }
+  const int G = 8;
+  const int group = w / (G * nb);
+  const int first = group * G;
+  const int gsize = min(nb - first, G);
+  const int in = w - group * G * nb;
+  bi = first + in % gsize;
+  bj = in / gsize;
+}
+
+int launch_table(pycd_ctx* ctx, int precision, const double* pos, int n,
+                 const double* kv, const double* sw, int K,
+                 const MicGeometry& geo, const EwaldScalars& es, double* out,
+                 unsigned long long* pairs, hipEvent_t after_sf);
+
+}  // namespace pycd

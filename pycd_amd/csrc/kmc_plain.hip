@@ -1,0 +1,676 @@
+// Plain (one-trajectory) KMC loop of PyCD's Run.do_kmc_steps on gfx950.
+//
+// Per hop (PyCD/core.py:2787-2861, rates from :1989-2050):
+//   dG0_p = 2 q_s [ q.(A[n,:] - A[s,:]) + q_s (A[s,s] - A[s,n]) ] + E[n] - E[s]
+//   dGs_p = ((lambda + dG0)^2 / (4 lambda) - V_ab) - 0.5 F.d
+//   k_p   = nu * e**(-dGs / kT)
+//   select the first p with cumsum(k / k_tot)_p > u1; t -= ln(u2) / k_tot
+//
+// The N-long dot product is never formed.  With q = q_lattice + sum_c q_c
+// e_{occ_c},  q.A[x,:] = phi_L[x] + psi[x],  where phi_L = A q_lattice is
+// computed once per table and psi[x] = sum_c q_c A[x, occ_c] is gathered
+// each hop for the sites x that matter: every carrier's site and its z hop
+// targets.  That is n_c (1 + z) x n_c table reads per hop instead of
+// n_c z x N, with the same value of q.(A[n,:] - A[s,:]) up to rounding.
+//
+// One persistent 256-thread workgroup runs the trajectory (the hop chain is
+// serial).  Phases per hop, separated by workgroup barriers:
+//   A  psi gathers                                    (all threads)
+//   B  dG0, Marcus/field rate per process             (all threads)
+//   C  k_tot, wavefront prefix scan of k/k_tot,
+//      selection, clock, state update                 (wavefront 0)
+//   D  path-step rows written for [start, end)        (all threads)
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace pycd {
+
+struct KmcScalars {
+  double t;
+  double energy;
+  int64_t start;
+  int64_t hops;
+  int32_t finished;
+  int32_t no_select;  // hops where rounding left no cum > u1 (last taken)
+};
+
+struct KmcDev {
+  const double* A;
+  const double* phiL;
+  const int32_t* nbr;
+  const double* vec;
+  const double* lam;
+  const double* vab;
+  const double* tself;
+  const double* E;
+  const double* cq;
+  int64_t N;
+  int32_t nc, z, max_z, field_on;
+  double field[3];
+  double nu, kT, dt;
+  int64_t P;
+  // state
+  int32_t* occ;
+  double* disp;
+  double* base;
+  double* drift;
+  double* rows;
+  double* epath;
+  double* dpath;
+  KmcScalars* sc;
+};
+
+constexpr int kThreads = 256;
+constexpr double kE = 2.718281828459045;  // numpy's e
+
+__device__ inline double wave_incl_scan(double v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(kThreads)
+kmc_plain_kernel(KmcDev d, const double* __restrict__ U, int64_t n_pairs,
+                 int64_t max_hops, double* __restrict__ hop_time,
+                 int32_t* __restrict__ hop_proc, int32_t* __restrict__ status) {
+  extern __shared__ double smem[];
+  const int nc = d.nc, z = d.z, R = nc * (1 + z), nproc = nc * z;
+  double* psi = smem;                 // R
+  double* kk = psi + R;               // nproc
+  double* dg = kk + nproc;            // nproc
+  double* disp = dg + nproc;          // 3 nc
+  double* base = disp + 3 * nc;       // 3 nc
+  double* drift = base + 3 * nc;      // 3 nc
+  int32_t* occ = reinterpret_cast<int32_t*>(drift + 3 * nc);  // nc
+  __shared__ double sh_t, sh_energy, sh_dg;
+  __shared__ int64_t sh_start, sh_end, sh_hops, sh_row_lo, sh_row_hi,
+      sh_dp_hi;
+  __shared__ int32_t sh_stop, sh_noselect;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  for (int c = tid; c < nc; c += kThreads) occ[c] = d.occ[c];
+  for (int k = tid; k < 3 * nc; k += kThreads) {
+    disp[k] = d.disp[k];
+    base[k] = d.base[k];
+    drift[k] = d.drift[k];
+  }
+  if (tid == 0) {
+    sh_t = d.sc->t;
+    sh_energy = d.sc->energy;
+    sh_start = d.sc->start;
+    sh_hops = d.sc->hops;
+    sh_noselect = d.sc->no_select;
+    sh_stop = d.sc->finished ? 1 : 0;
+  }
+  __syncthreads();
+  int64_t h = 0;  // hops made in this call
+  int stop = sh_stop;
+  int code = stop ? 0 : -1;
+
+  while (!stop) {
+    if (max_hops > 0 && sh_hops >= max_hops) {
+      code = 0;
+      break;
+    }
+    if (h >= n_pairs) {
+      code = 1;  // need uniforms
+      break;
+    }
+    // ---- A: psi[x] = sum_c q_c A[x, occ_c] for carrier sites and targets
+    for (int r = tid; r < R; r += kThreads) {
+      const int c = r / (1 + z);
+      const int slot = r - c * (1 + z);
+      const int s = occ[c];
+      const int x = slot == 0 ? s : d.nbr[int64_t(s) * d.max_z + slot - 1];
+      const double* row = d.A + int64_t(x) * d.N;
+      double acc = 0.0;
+      int c2 = 0;
+      for (; c2 + 8 <= nc; c2 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = row[occ[c2 + u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = fma(d.cq[c2 + u], v[u], acc);
+      }
+      for (; c2 < nc; ++c2) acc = fma(d.cq[c2], row[occ[c2]], acc);
+      psi[r] = acc;
+    }
+    __syncthreads();
+    // ---- B: rates
+    for (int p = tid; p < nproc; p += kThreads) {
+      const int c = p / z;
+      const int j = p - c * z;
+      const int s = occ[c];
+      const int64_t sj = int64_t(s) * d.max_z + j;
+      const int n = d.nbr[sj];
+      const double qs = d.cq[c];
+      const double term01 = (d.phiL[n] - d.phiL[s]) +
+                            (psi[c * (1 + z) + 1 + j] - psi[c * (1 + z)]);
+      const double term02 = qs * d.tself[sj];
+      const double g0 = 2 * qs * (term01 + term02) + (d.E[n] - d.E[s]);
+      const double lam = d.lam[sj];
+      double shift = 0.0;
+      if (d.field_on)
+        shift = 0.5 * (d.field[0] * d.vec[3 * sj] +
+                       d.field[1] * d.vec[3 * sj + 1] +
+                       d.field[2] * d.vec[3 * sj + 2]);
+      const double gs = (((lam + g0) * (lam + g0) / (4 * lam)) - d.vab[sj]) -
+                        shift;
+      kk[p] = d.nu * pow(kE, -gs / d.kT);
+      dg[p] = g0;
+    }
+    __syncthreads();
+    // ---- C: selection, clock and state update on wavefront 0
+    if (tid < 64) {
+      const int per = (nproc + 63) / 64;
+      const int p0 = lane * per;
+      const int p1 = min(p0 + per, nproc);
+      double ksum = 0.0;
+      for (int p = p0; p < p1; ++p) ksum += kk[p];
+      const double k_tot = __shfl(wave_incl_scan(ksum, lane), 63, 64);
+      const double u1 = U[2 * h];
+      const double u2 = U[2 * h + 1];
+      double xsum = 0.0;
+      for (int p = p0; p < p1; ++p) xsum += kk[p] / k_tot;
+      const double xincl = wave_incl_scan(xsum, lane);
+      double cum = __shfl_up(xincl, 1, 64);
+      if (lane == 0) cum = 0.0;
+      int found = -1;
+      for (int p = p0; p < p1; ++p) {
+        cum += kk[p] / k_tot;
+        if (cum > u1) {
+          found = p;
+          break;
+        }
+      }
+      const unsigned long long mask = __ballot(found >= 0);
+      int sel;
+      if (mask) {
+        const int src = __ffsll(static_cast<long long>(mask)) - 1;
+        sel = __shfl(found, src, 64);
+      } else {
+        sel = nproc - 1;
+      }
+      if (lane == 0) {
+        if (!mask) ++sh_noselect;
+        const int c = sel / z;
+        const int j = sel - c * z;
+        const int s_old = occ[c];
+        const int64_t sj = int64_t(s_old) * d.max_z + j;
+        const int s_new = d.nbr[sj];
+        double t = sh_t;
+        int64_t end;
+        if (u2 > 0.0) {
+          t -= log(u2) / k_tot;
+          const double q = t / d.dt;
+          end = q < 9.0e18 ? int64_t(q) : d.P;
+        } else {
+          t = INFINITY;
+          end = d.P;
+        }
+        sh_t = t;
+        const double g = dg[sel];
+        sh_dg = g;
+        // delg_0 path: [start, end) clipped to P (numpy slice semantics)
+        sh_dp_hi = min(end, d.P);
+        occ[c] = s_new;
+        disp[3 * c] += d.vec[3 * sj];
+        disp[3 * c + 1] += d.vec[3 * sj + 1];
+        disp[3 * c + 2] += d.vec[3 * sj + 2];
+        if (d.field_on) {
+          drift[3 * c] += d.vec[3 * sj] * kk[sel];
+          drift[3 * c + 1] += d.vec[3 * sj + 1] * kk[sel];
+          drift[3 * c + 2] += d.vec[3 * sj + 2] * kk[sel];
+        }
+        sh_energy += g;
+        if (hop_time) hop_time[h] = t;
+        if (hop_proc) hop_proc[h] = sel;
+        sh_row_lo = sh_start;
+        sh_row_hi = sh_start;
+        if (end >= sh_start + 1) {
+          if (end >= d.P) end = d.P;
+          sh_row_hi = end;
+        }
+        sh_end = end;
+        sh_hops += 1;
+      }
+    }
+    __syncthreads();
+    // ---- D: path-step outputs on all threads
+    {
+      const int64_t lo = sh_row_lo, hi = sh_row_hi;
+      const double g = sh_dg;
+      for (int64_t r = lo + tid; r < sh_dp_hi; r += kThreads) d.dpath[r] = g;
+      if (hi > lo) {
+        const int w = 3 * nc;
+        const double e = sh_energy;
+        for (int64_t idx = tid; idx < (hi - lo) * w; idx += kThreads) {
+          const int64_t r = lo + idx / w;
+          const int k = int(idx % w);
+          d.rows[r * w + k] = base[k] + disp[k];
+        }
+        for (int64_t r = lo + tid; r < hi; r += kThreads) d.epath[r] = e;
+      }
+    }
+    __syncthreads();
+    if (sh_row_hi > sh_row_lo) {
+      for (int k = tid; k < 3 * nc; k += kThreads) {
+        base[k] += disp[k];
+        disp[k] = 0.0;
+      }
+      if (tid == 0) sh_start = sh_row_hi;
+    }
+    ++h;
+    // loop condition of the reference: continue while end < P
+    if (tid == 0) sh_stop = (sh_row_hi > sh_row_lo && sh_row_hi >= d.P) ? 1 : 0;
+    __syncthreads();
+    stop = sh_stop;
+    if (stop) code = 0;
+  }
+  // ---- write the state back
+  for (int c = tid; c < nc; c += kThreads) d.occ[c] = occ[c];
+  for (int k = tid; k < 3 * nc; k += kThreads) {
+    d.disp[k] = disp[k];
+    d.base[k] = base[k];
+    d.drift[k] = drift[k];
+  }
+  if (tid == 0) {
+    d.sc->t = sh_t;
+    d.sc->energy = sh_energy;
+    d.sc->start = sh_start;
+    d.sc->hops = sh_hops;
+    d.sc->no_select = sh_noselect;
+    if (stop) d.sc->finished = 1;
+    status[0] = code;
+    status[1] = int32_t(h);
+  }
+}
+
+// phi_L[i] = sum_j A[i, j] q_L[j]: one wavefront per row.
+__global__ void __launch_bounds__(256)
+row_dot_kernel(const double* __restrict__ A, const double* __restrict__ q,
+               int64_t N, double* __restrict__ out) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t i = int64_t(blockIdx.x) * 4 + wave; i < N;
+       i += int64_t(gridDim.x) * 4) {
+    const double* row = A + i * N;
+    double acc = 0.0;
+    for (int64_t j = lane; j < N; j += 64) acc = fma(row[j], q[j], acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) out[i] = acc;
+  }
+}
+
+// tself[s, j] = A[s, s] - A[s, nbr(s, j)] for valid slots.
+__global__ void self_term_kernel(const double* __restrict__ A,
+                                 const int32_t* __restrict__ nbr, int64_t N,
+                                 int max_z, double* __restrict__ tself) {
+  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= N * max_z) return;
+  const int64_t s = idx / max_z;
+  const int n = nbr[idx];
+  tself[idx] = n >= 0 ? A[s * N + s] - A[s * N + n] : 0.0;
+}
+
+// Initial energy pieces: e[0] = q_L . phi_L is precomputed on the host side
+// of the session; this kernel adds the carrier terms
+// 2 sum_c q_c phi_L[occ_c] + sum_{c,c'} q_c q_c' A[occ_c, occ_c'].
+__global__ void carrier_energy_kernel(const double* __restrict__ A,
+                                      const double* __restrict__ phiL,
+                                      const double* __restrict__ cq,
+                                      const int32_t* __restrict__ occ,
+                                      int nc, int64_t N,
+                                      double* __restrict__ out) {
+  __shared__ double part[256];
+  double acc = 0.0;
+  for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+    const double* row = A + int64_t(occ[c]) * N;
+    double inner = 0.0;
+    for (int c2 = 0; c2 < nc; ++c2) inner = fma(cq[c2], row[occ[c2]], inner);
+    acc += cq[c] * (2.0 * phiL[occ[c]] + inner);
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
+__global__ void dot_kernel(const double* __restrict__ a,
+                           const double* __restrict__ b, int64_t n,
+                           double* __restrict__ out) {
+  __shared__ double part[256];
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += blockDim.x)
+    acc = fma(a[k], b[k], acc);
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
+}  // namespace pycd
+
+using namespace pycd;
+
+struct pycd_kmc {
+  pycd_ctx* ctx = nullptr;
+  pycd_kmc_problem pb{};
+  DevBuf<double> A, phiL, vec, lam, vab, tself, E, cq, qL;
+  DevBuf<int32_t> nbr, occ, status;
+  DevBuf<double> disp, base, drift, rows, epath, dpath, tmp;
+  DevBuf<KmcScalars> sc;
+  DevBuf<double> ubuf, tbuf;
+  DevBuf<int32_t> pbuf;
+  double qLAqL = 0.0;
+  size_t smem = 0;
+  bool begun = false;
+};
+
+namespace {
+
+int check_problem(pycd_ctx* ctx, const pycd_kmc_problem* pb) {
+  if (!pb) return fail(ctx, PYCD_ERR_ARG, "null problem");
+  if (pb->n_sites <= 0 || pb->n_sites > (int64_t(1) << 31) - 1)
+    return fail(ctx, PYCD_ERR_ARG, "bad n_sites");
+  if (pb->n_carriers <= 0 || pb->z <= 0 || pb->max_z < pb->z)
+    return fail(ctx, PYCD_ERR_ARG, "bad carrier/slot counts");
+  if (!pb->table || !pb->lattice_q || !pb->nbr_site || !pb->nbr_vec ||
+      !pb->nbr_lambda || !pb->nbr_vab || !pb->site_energy || !pb->carrier_q)
+    return fail(ctx, PYCD_ERR_ARG, "null table pointer");
+  if (!(pb->time_interval > 0) || pb->n_path < 1 || !(pb->kT > 0))
+    return fail(ctx, PYCD_ERR_ARG, "bad time grid or temperature");
+  return PYCD_OK;
+}
+
+}  // namespace
+
+extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
+                               pycd_kmc** out) {
+  if (!ctx || !out) return PYCD_ERR_ARG;
+  *out = nullptr;
+  int rc = check_problem(ctx, pb);
+  if (rc) return rc;
+  DeviceGuard guard(ctx->device);
+  const int64_t N = pb->n_sites;
+  const int nc = pb->n_carriers, z = pb->z, mz = pb->max_z;
+  const size_t smem = sizeof(double) * (size_t(nc) * (1 + z) + 2 * size_t(nc) * z +
+                                        9 * size_t(nc)) +
+                      sizeof(int32_t) * nc + 16;
+  if (smem > 60 * 1024)
+    return fail(ctx, PYCD_ERR_ARG,
+                "n_c=%d, z=%d need %zu B of LDS (limit 60 KiB)", nc, z, smem);
+  // Host-side validation of the slot table (every index the kernel reads).
+  {
+    std::vector<int32_t> nb(size_t(N) * mz);
+    if (is_device_ptr(pb->nbr_site)) {
+      PYCD_HIP(ctx, hipMemcpy(nb.data(), pb->nbr_site, nb.size() * 4,
+                              hipMemcpyDeviceToHost));
+    } else {
+      std::memcpy(nb.data(), pb->nbr_site, nb.size() * 4);
+    }
+    for (size_t k = 0; k < nb.size(); ++k)
+      if (nb[k] < -1 || nb[k] >= N)
+        return fail(ctx, PYCD_ERR_ARG, "nbr_site[%zu]=%d out of range", k,
+                    nb[k]);
+  }
+  pycd_kmc* s = new (std::nothrow) pycd_kmc();
+  if (!s) return PYCD_ERR_NOMEM;
+  s->ctx = ctx;
+  s->pb = *pb;
+  s->smem = smem;
+  hipStream_t st = ctx->stream;
+  auto bail = [&](int code) {
+    delete s;
+    return code;
+  };
+#define S_HIP(expr)                                                         \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return bail(fail(ctx, PYCD_ERR_HIP, "%s: %s", #expr,                  \
+                       hipGetErrorString(e_)));                             \
+  } while (0)
+  const size_t NM = size_t(N) * mz;
+  const int64_t P = pb->n_path;
+  S_HIP(s->A.alloc(size_t(N) * N));
+  S_HIP(s->phiL.alloc(N));
+  S_HIP(s->qL.alloc(N));
+  S_HIP(s->nbr.alloc(NM));
+  S_HIP(s->vec.alloc(NM * 3));
+  S_HIP(s->lam.alloc(NM));
+  S_HIP(s->vab.alloc(NM));
+  S_HIP(s->tself.alloc(NM));
+  S_HIP(s->E.alloc(N));
+  S_HIP(s->cq.alloc(nc));
+  S_HIP(s->occ.alloc(nc));
+  S_HIP(s->status.alloc(2));
+  S_HIP(s->disp.alloc(3 * nc));
+  S_HIP(s->base.alloc(3 * nc));
+  S_HIP(s->drift.alloc(3 * nc));
+  S_HIP(s->rows.alloc(size_t(P) * 3 * nc));
+  S_HIP(s->epath.alloc(P));
+  S_HIP(s->dpath.alloc(P));
+  S_HIP(s->tmp.alloc(2));
+  S_HIP(s->sc.alloc(1));
+  S_HIP(upload(s->A.ptr, pb->table, size_t(N) * N, st));
+  S_HIP(upload(s->qL.ptr, pb->lattice_q, N, st));
+  S_HIP(upload(s->nbr.ptr, pb->nbr_site, NM, st));
+  S_HIP(upload(s->vec.ptr, pb->nbr_vec, NM * 3, st));
+  S_HIP(upload(s->lam.ptr, pb->nbr_lambda, NM, st));
+  S_HIP(upload(s->vab.ptr, pb->nbr_vab, NM, st));
+  S_HIP(upload(s->E.ptr, pb->site_energy, N, st));
+  S_HIP(upload(s->cq.ptr, pb->carrier_q, nc, st));
+  {
+    const int blocks = int(std::min<int64_t>((N + 3) / 4, 8192));
+    row_dot_kernel<<<blocks, 256, 0, st>>>(s->A.ptr, s->qL.ptr, N, s->phiL.ptr);
+    S_HIP(hipGetLastError());
+    self_term_kernel<<<int((NM + 255) / 256), 256, 0, st>>>(s->A.ptr, s->nbr.ptr,
+                                                           N, mz, s->tself.ptr);
+    S_HIP(hipGetLastError());
+    dot_kernel<<<1, 256, 0, st>>>(s->qL.ptr, s->phiL.ptr, N, s->tmp.ptr);
+    S_HIP(hipGetLastError());
+    S_HIP(hipMemcpyAsync(&s->qLAqL, s->tmp.ptr, sizeof(double),
+                         hipMemcpyDeviceToHost, st));
+    S_HIP(hipStreamSynchronize(st));
+  }
+#undef S_HIP
+  *out = s;
+  return PYCD_OK;
+}
+
+extern "C" void pycd_kmc_destroy(pycd_kmc* s) {
+  if (!s) return;
+  DeviceGuard guard(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+}
+
+extern "C" int pycd_kmc_begin(pycd_kmc* s, const int64_t* occupancy,
+                              double ewald_neut) {
+  if (!s || !occupancy) return PYCD_ERR_ARG;
+  pycd_ctx* ctx = s->ctx;
+  DeviceGuard guard(ctx->device);
+  const pycd_kmc_problem& pb = s->pb;
+  const int nc = pb.n_carriers;
+  std::vector<int32_t> occ(nc);
+  {
+    // occupancy comes from the host API; validate every site.
+    std::vector<int32_t> nb(size_t(pb.n_sites) * pb.max_z);
+    PYCD_HIP(ctx, hipMemcpy(nb.data(), s->nbr.ptr, nb.size() * 4,
+                            hipMemcpyDeviceToHost));
+    for (int c = 0; c < nc; ++c) {
+      const int64_t v = occupancy[c];
+      if (v < 0 || v >= pb.n_sites)
+        return fail(ctx, PYCD_ERR_ARG, "occupancy[%d]=%lld out of range", c,
+                    (long long)v);
+      for (int j = 0; j < pb.z; ++j)
+        if (nb[size_t(v) * pb.max_z + j] < 0)
+          return fail(ctx, PYCD_ERR_ARG,
+                      "carrier %d sits on site %lld with fewer than z=%d hop "
+                      "slots",
+                      c, (long long)v, pb.z);
+      occ[c] = int32_t(v);
+    }
+  }
+  hipStream_t st = ctx->stream;
+  const int64_t P = pb.n_path;
+  PYCD_HIP(ctx, hipMemcpyAsync(s->occ.ptr, occ.data(), 4 * nc,
+                               hipMemcpyHostToDevice, st));
+  PYCD_HIP(ctx, hipMemsetAsync(s->disp.ptr, 0, 24 * nc, st));
+  PYCD_HIP(ctx, hipMemsetAsync(s->base.ptr, 0, 24 * nc, st));
+  PYCD_HIP(ctx, hipMemsetAsync(s->drift.ptr, 0, 24 * nc, st));
+  PYCD_HIP(ctx, hipMemsetAsync(s->rows.ptr, 0, 24 * size_t(nc) * P, st));
+  PYCD_HIP(ctx, hipMemsetAsync(s->epath.ptr, 0, 8 * size_t(P), st));
+  PYCD_HIP(ctx, hipMemsetAsync(s->dpath.ptr, 0, 8 * size_t(P), st));
+  carrier_energy_kernel<<<1, 256, 0, st>>>(s->A.ptr, s->phiL.ptr, s->cq.ptr,
+                                          s->occ.ptr, nc, pb.n_sites,
+                                          s->tmp.ptr);
+  PYCD_HIP(ctx, hipGetLastError());
+  double carrier_part = 0.0;
+  PYCD_HIP(ctx, hipMemcpyAsync(&carrier_part, s->tmp.ptr, sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+  PYCD_HIP(ctx, hipStreamSynchronize(st));
+  KmcScalars sc{};
+  sc.t = 0.0;
+  sc.energy = ewald_neut + (s->qLAqL + carrier_part);
+  sc.start = 1;
+  sc.hops = 0;
+  sc.finished = P <= 1 ? 1 : 0;
+  sc.no_select = 0;
+  PYCD_HIP(ctx, hipMemcpyAsync(s->sc.ptr, &sc, sizeof(sc),
+                               hipMemcpyHostToDevice, st));
+  PYCD_HIP(ctx, hipMemcpyAsync(s->epath.ptr, &sc.energy, sizeof(double),
+                               hipMemcpyHostToDevice, st));
+  PYCD_HIP(ctx, hipStreamSynchronize(st));
+  s->begun = true;
+  return PYCD_OK;
+}
+
+extern "C" int pycd_kmc_advance(pycd_kmc* s, const double* uniforms,
+                                int64_t n_pairs, int64_t max_hops,
+                                double* hop_time, int32_t* hop_proc,
+                                int64_t* hops_out) {
+  if (!s) return PYCD_ERR_ARG;
+  pycd_ctx* ctx = s->ctx;
+  if (!s->begun) return fail(ctx, PYCD_ERR_STATE, "pycd_kmc_begin not called");
+  if (n_pairs < 0 || (n_pairs > 0 && !uniforms))
+    return fail(ctx, PYCD_ERR_ARG, "bad uniform buffer");
+  if (n_pairs > (int64_t(1) << 30))
+    return fail(ctx, PYCD_ERR_ARG, "at most 2^30 uniform pairs per call");
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = ctx->stream;
+  const double* dU = uniforms;
+  if (n_pairs > 0 && !is_device_ptr(uniforms)) {
+    if (s->ubuf.count < size_t(2 * n_pairs))
+      PYCD_HIP(ctx, s->ubuf.alloc(size_t(2 * n_pairs)));
+    PYCD_HIP(ctx, hipMemcpyAsync(s->ubuf.ptr, uniforms, 16 * size_t(n_pairs),
+                                 hipMemcpyHostToDevice, st));
+    dU = s->ubuf.ptr;
+  }
+  double* dT = hop_time;
+  int32_t* dP = hop_proc;
+  const bool t_host = hop_time && !is_device_ptr(hop_time);
+  const bool p_host = hop_proc && !is_device_ptr(hop_proc);
+  if (t_host) {
+    if (s->tbuf.count < size_t(n_pairs))
+      PYCD_HIP(ctx, s->tbuf.alloc(size_t(std::max<int64_t>(n_pairs, 1))));
+    dT = s->tbuf.ptr;
+  }
+  if (p_host) {
+    if (s->pbuf.count < size_t(n_pairs))
+      PYCD_HIP(ctx, s->pbuf.alloc(size_t(std::max<int64_t>(n_pairs, 1))));
+    dP = s->pbuf.ptr;
+  }
+  const pycd_kmc_problem& pb = s->pb;
+  KmcDev d;
+  d.A = s->A.ptr;
+  d.phiL = s->phiL.ptr;
+  d.nbr = s->nbr.ptr;
+  d.vec = s->vec.ptr;
+  d.lam = s->lam.ptr;
+  d.vab = s->vab.ptr;
+  d.tself = s->tself.ptr;
+  d.E = s->E.ptr;
+  d.cq = s->cq.ptr;
+  d.N = pb.n_sites;
+  d.nc = pb.n_carriers;
+  d.z = pb.z;
+  d.max_z = pb.max_z;
+  d.field_on = (pb.field[0] != 0.0 || pb.field[1] != 0.0 || pb.field[2] != 0.0);
+  for (int a = 0; a < 3; ++a) d.field[a] = pb.field[a];
+  d.nu = pb.nu;
+  d.kT = pb.kT;
+  d.dt = pb.time_interval;
+  d.P = pb.n_path;
+  d.occ = s->occ.ptr;
+  d.disp = s->disp.ptr;
+  d.base = s->base.ptr;
+  d.drift = s->drift.ptr;
+  d.rows = s->rows.ptr;
+  d.epath = s->epath.ptr;
+  d.dpath = s->dpath.ptr;
+  d.sc = s->sc.ptr;
+  kmc_plain_kernel<<<1, kThreads, s->smem, st>>>(d, dU, n_pairs, max_hops, dT,
+                                                 dP, s->status.ptr);
+  PYCD_HIP(ctx, hipGetLastError());
+  int32_t status[2] = {-1, 0};
+  PYCD_HIP(ctx, hipMemcpyAsync(status, s->status.ptr, sizeof(status),
+                               hipMemcpyDeviceToHost, st));
+  PYCD_HIP(ctx, hipStreamSynchronize(st));
+  const int64_t made = status[1];
+  if (t_host && made > 0)
+    PYCD_HIP(ctx, hipMemcpy(hop_time, dT, 8 * size_t(made),
+                            hipMemcpyDeviceToHost));
+  if (p_host && made > 0)
+    PYCD_HIP(ctx, hipMemcpy(hop_proc, dP, 4 * size_t(made),
+                            hipMemcpyDeviceToHost));
+  if (hops_out) *hops_out = made;
+  if (status[0] < 0) return fail(ctx, PYCD_ERR_STATE, "kernel did not report");
+  return status[0] == 1 ? PYCD_NEED_UNIFORMS : PYCD_OK;
+}
+
+extern "C" int pycd_kmc_read(pycd_kmc* s, double* unwrapped, double* energy,
+                             double* delg_0, double* drift, int64_t* occupancy,
+                             double scalars[4]) {
+  if (!s) return PYCD_ERR_ARG;
+  pycd_ctx* ctx = s->ctx;
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = ctx->stream;
+  const pycd_kmc_problem& pb = s->pb;
+  const int nc = pb.n_carriers;
+  const int64_t P = pb.n_path;
+  if (unwrapped)
+    PYCD_HIP(ctx, download(unwrapped, s->rows.ptr, size_t(P) * 3 * nc, st));
+  if (energy) PYCD_HIP(ctx, download(energy, s->epath.ptr, size_t(P), st));
+  if (delg_0) PYCD_HIP(ctx, download(delg_0, s->dpath.ptr, size_t(P), st));
+  if (drift) PYCD_HIP(ctx, download(drift, s->drift.ptr, 3 * size_t(nc), st));
+  KmcScalars sc{};
+  PYCD_HIP(ctx, hipMemcpyAsync(&sc, s->sc.ptr, sizeof(sc),
+                               hipMemcpyDeviceToHost, st));
+  std::vector<int32_t> occ(nc);
+  PYCD_HIP(ctx, hipMemcpyAsync(occ.data(), s->occ.ptr, 4 * nc,
+                               hipMemcpyDeviceToHost, st));
+  PYCD_HIP(ctx, hipStreamSynchronize(st));
+  if (occupancy)
+    for (int c = 0; c < nc; ++c) occupancy[c] = occ[c];
+  if (scalars) {
+    scalars[0] = sc.t;
+    scalars[1] = sc.energy;
+    scalars[2] = double(sc.start);
+    scalars[3] = double(sc.hops);
+  }
+  return PYCD_OK;
+}

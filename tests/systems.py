@@ -1,0 +1,49 @@
+"""Builders shared by the tests: the reference's two shipped examples."""
+import types
+
+import numpy as np
+
+from pycd_amd.core import Material, Neighbors, System, Run, ReturnValues
+from tests.conftest import golden_dir, load_configs
+
+
+def build_example(material):
+    cfg, sim = load_configs(material)
+    d = golden_dir(material)
+    mat = Material(ReturnValues(**cfg))
+    nb = Neighbors(mat, np.array(sim["system_size"]), np.array(sim["pbc"]))
+    table = np.load(d / "precomputed_array.npy")
+    pairwise = np.load(d / "pairwise_min_image_vector_data.npy")
+    system = System(mat, nb, None, pairwise, cfg["alpha"], cfg["r_cut"],
+                    cfg["k_cut"], cfg["precision_parameters"], [], [])
+    run = Run(system, table, sim["temp"], sim["ion_charge_type"],
+              sim["species_charge_type"], sim["n_traj"], sim["t_final"],
+              sim["time_interval"], np.array(sim["species_count"]),
+              sim["initial_occupancy"], sim["relative_energies"],
+              sim["external_field"], sim["doping"])
+    # Site order of the shipped table: the reference's z-sort broke ties of
+    # equal-z sites differently from this NumPy, so positions are rebuilt
+    # from the shipped minimum-image vectors of site 0 (a pure function of
+    # the shipped data; site 0 is a hopping site with a unique z).
+    shipped_pos = nb.bulk_sites.cell_coordinates[0] + pairwise[0]
+    return types.SimpleNamespace(
+        name=material, cfg=cfg, sim=sim, material=mat, neighbors=nb,
+        system=system, run=run, table=table, pairwise=pairwise,
+        shipped_pos=shipped_pos, dir=d,
+        time_data=np.load(d / "time_data.npy"),
+        unwrapped=np.load(d / "unwrapped_traj.npy"),
+        rnd_dump=(d / "initial_rnd_state.dump").read_bytes())
+
+
+def oracle_problem(run, table=None, field=None):
+    """oracle.kmc.Problem from a pycd_amd Run (same host tables)."""
+    from oracle import kmc
+    return kmc.Problem(
+        run.precomputed_array if table is None else table,
+        run.base_charge_config()[:, 0], run.nbr_count, run.nbr_site,
+        run.nbr_vec, run.nbr_lambda, run.nbr_vab,
+        run.system_relative_energies, run.species_charge_list,
+        (run.electric_field if run.electric_field_active else np.zeros(3))
+        if field is None else field,
+        run.material.vn, run.temp, run.time_interval,
+        int(run.t_final / run.time_interval) + 1)

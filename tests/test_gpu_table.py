@@ -1,0 +1,75 @@
+"""HIP Ewald table vs the shipped tables and the oracle (FP64 mode)."""
+import numpy as np
+import pytest
+
+from oracle import ewald
+from pycd_amd import _native, workloads
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-12  # max |A_gpu - A_ref| / max |A_ref|
+
+
+def gpu_table(system, pos, precision="f64"):
+    return _native.pair_table(
+        pos, system.neighbors.simulation_cell_matrix,
+        system.translational_matrix, system.pbc, system.alpha, system.r_cut,
+        system.k_cut, system.material.dielectric_constant, precision)
+
+
+@pytest.mark.parametrize("name", ["hematite", "bvo"])
+def test_matches_shipped_table(name, request):
+    ex = request.getfixturevalue(name)
+    table, rep = gpu_table(ex.system, ex.shipped_pos)
+    err = np.max(np.abs(table - ex.table)) / np.max(np.abs(ex.table))
+    print(name, "rel err", err, rep)
+    assert err <= REL_TOL
+    assert rep["n_kvec"] > 0 and rep["n_real_pairs"] >= len(table)
+
+
+def test_small_corundum_vs_oracle():
+    w = workloads.corundum(3)
+    pos = w.neighbors.bulk_sites.cell_coordinates
+    table, rep = gpu_table(w.system, pos)
+    assert rep["n_kvec"] == 1053 and list(rep["k_max"]) == [6, 6, 17]
+    cell = w.neighbors.simulation_cell_matrix
+    pw = ewald.pairwise_vectors(pos, cell, [1, 1, 1])
+    ref = ewald.pair_table(pw, w.material.lattice_matrix, w.size,
+                           w.system.alpha, w.system.r_cut, w.system.k_cut, 1.0)
+    err = np.max(np.abs(table - ref)) / np.max(np.abs(ref))
+    print("N=324 rel err", err)
+    assert err <= REL_TOL
+
+
+def test_large_corundum_rows_and_symmetry():
+    """8x8x8 (N = 6144): symmetry and 4 full rows against the oracle."""
+    w = workloads.corundum(8)
+    pos = w.neighbors.bulk_sites.cell_coordinates
+    table, rep = gpu_table(w.system, pos)
+    assert table.shape == (6144, 6144)
+    scale = np.max(np.abs(table))
+    assert np.max(np.abs(table - table.T)) <= 1e-13 * scale
+    rows = [0, 1, 3071, 6143]
+    ref = ewald.table_rows(pos, rows, w.material.lattice_matrix, w.size,
+                           [1, 1, 1], w.system.alpha, w.system.r_cut,
+                           w.system.k_cut, 1.0)
+    err = np.max(np.abs(table[rows] - ref)) / scale
+    print("N=6144 rows rel err", err, rep)
+    assert err <= REL_TOL
+
+
+def test_system_api_writes_log(tmp_path, hematite):
+    """System.get_precomputed_array: table + log with alpha on line 4."""
+    table, dst = hematite.system.get_precomputed_array(tmp_path, 0, 0)
+    assert dst == tmp_path
+    lines = (tmp_path / "precomputed_array.log").read_text().splitlines()
+    assert float(lines[3][7:16]) == float(f"{hematite.cfg['alpha']:.3e}")
+    pos = hematite.neighbors.bulk_sites.cell_coordinates
+    cell = hematite.neighbors.simulation_cell_matrix
+    pw = ewald.pairwise_vectors(pos, cell, hematite.sim["pbc"])
+    al, rc, kc = ewald.ewald_units(hematite.cfg["alpha"], hematite.cfg["r_cut"],
+                                   hematite.cfg["k_cut"])
+    ref = ewald.pair_table(pw, hematite.material.lattice_matrix,
+                           np.array(hematite.sim["system_size"]), al, rc, kc,
+                           hematite.cfg["dielectric_constant"])
+    assert np.max(np.abs(table - ref)) / np.max(np.abs(ref)) <= REL_TOL
