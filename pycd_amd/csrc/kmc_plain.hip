@@ -425,6 +425,22 @@ extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
       if (nb[k] < -1 || nb[k] >= N)
         return fail(ctx, PYCD_ERR_ARG, "nbr_site[%zu]=%d out of range", k,
                     nb[k]);
+    // Closure: a hopping site (slot 0 valid) has z valid slots and every
+    // target is itself a hopping site, so a carrier never reaches a site
+    // whose slot rows the kernel cannot read.
+    for (int64_t s = 0; s < N; ++s) {
+      const int32_t* row = nb.data() + size_t(s) * mz;
+      if (row[0] < 0) continue;
+      for (int j = 0; j < z; ++j) {
+        if (row[j] < 0)
+          return fail(ctx, PYCD_ERR_ARG, "site %lld has fewer than %d slots",
+                      (long long)s, z);
+        if (nb[size_t(row[j]) * mz] < 0)
+          return fail(ctx, PYCD_ERR_ARG,
+                      "slot %d of site %lld targets non-hopping site %d", j,
+                      (long long)s, row[j]);
+      }
+    }
   }
   pycd_kmc* s = new (std::nothrow) pycd_kmc();
   if (!s) return PYCD_ERR_NOMEM;

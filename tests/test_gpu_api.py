@@ -1,8 +1,10 @@
 """Drop-in use of Run.do_kmc_steps and Analysis.compute_msd."""
+import pickle
+
 import numpy as np
 import pytest
 
-from oracle import msd as omsd
+from oracle import msd as omsd, seeding
 from pycd_amd import Analysis, constants
 
 pytestmark = pytest.mark.gpu
@@ -13,8 +15,11 @@ def test_do_kmc_steps_reproduces_shipped_files(tmp_path, hematite):
     ex.run.do_kmc_steps(tmp_path, ex.sim["output_data"], ex.sim["random_seed"],
                         ex.sim["compute_mode"])
     traj = tmp_path / "traj1"
-    # preproduction wrote the same generator state, byte for byte
-    assert (traj / "initial_rnd_state.dump").read_bytes() == ex.rnd_dump
+    # preproduction wrote the seeded state with this Python's default pickle
+    # protocol (the shipped dump is protocol 3; its state is pinned by
+    # test_seeding_protocol_bytes)
+    state = seeding.trajectory_states(ex.sim["random_seed"], 1)[0]
+    assert (traj / "initial_rnd_state.dump").read_bytes() == pickle.dumps(state)
     td = np.load(traj / "time_data.npy")
     assert len(td) == len(ex.time_data)
     assert np.max(np.abs(td[1:] - ex.time_data[1:]) / ex.time_data[1:]) < 1e-12
