@@ -1,0 +1,233 @@
+"""Benchmark of the KMC hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode batched|plain]
+
+Headline workload (`batched`): 4096 independent trajectories in one
+persistent launch -- Fe-only corundum supercells with edge U{4..8} (768 to
+6144 sites), 1..64 electrons, |E| ~ U(0, 1e7) V/m along a random lattice
+direction, T ~ U(250, 400) K, 1e5 hops each, FP32 rates with an FP64
+prefix sum and clock, per-trajectory MSD reduced on the device.  One step
+= one pass over the whole batch (4.1e8 hops).  Inputs (tables, hop slots,
+generator states, initial occupancies) are resident in HBM before timing.
+
+Beside it the plain one-trajectory path is timed on its own workload
+(8x8x8 supercell, 6144 sites, 16 electrons, 1e6 V/m along c, 1e6 hops,
+FP64, uniforms resident in HBM), and the NumPy oracle on a bounded sample
+of the batched workload gives the CPU baseline.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
+    ap.add_argument("--n-traj", type=int, default=4096)
+    ap.add_argument("--hops", type=int, default=100_000)
+    ap.add_argument("--plain-hops", type=int, default=1_000_000)
+    ap.add_argument("--waves-per-cu", type=int, default=0,
+                    help="0: autotune")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def algorithmic_bytes_batched(trajs, fams, hops, starts, table_bytes):
+    """Bytes the batched algorithm must touch per launch.
+
+    Per hop of a trajectory with n_c carriers and z slots:
+      table gathers   3 (1 + z) n_c entries          (psi update + refresh)
+      per process     64 B slot record + 4 x 8 B (phi_L, E at both ends)
+    plus 3 n_c doubles per path row written.
+    """
+    total = 0.0
+    for k, t in enumerate(trajs):
+        z = fams[t["family"]].z
+        nc = t["n_carriers"]
+        per_hop = 3 * (1 + z) * nc * table_bytes + nc * z * (64 + 32)
+        total += per_hop * hops[k] + 24.0 * nc * starts[k]
+    return total
+
+
+def run_batched(args, log):
+    from pycd_amd import workloads, batch as B, tuning, constants
+    t0 = time.time()
+    fams, works, trajs, occs, words, lags = workloads.batched_config(
+        n_traj=args.n_traj, hops=args.hops)
+    log(f"batched setup: {len(fams)} families, {len(trajs)} trajectories "
+        f"({time.time() - t0:.1f}s)")
+    order = tuning.lpt_order(trajs, fams)
+    wpc = args.waves_per_cu
+    bt = B.KmcBatch(fams, table_precision="f64", rate_precision="f32")
+    if wpc <= 0:
+        wpc, table = tuning.autotune_waves(bt, trajs, occs, words, order,
+                                           log=log)
+    bt.setup(trajs, occs, words, order=order, msd_lags=lags,
+             msd_scale=1 / constants.ANG2BOHR, waves_per_cu=wpc)
+    for _ in range(args.warmup):
+        bt.run(use_graph=True)
+    times = []
+    for _ in range(args.steps):
+        times.append(bt.run(use_graph=True))
+    res = bt.results()
+    hops = res["hops"].astype(float)
+    total_hops = hops.sum()
+    ms = float(np.mean(times))
+    value = total_hops / (ms / 1e3)
+    # kernel-only timing (no graph) for the roofline of the dominant kernel
+    bt_k = bt.run(use_graph=False)
+    algo = algorithmic_bytes_batched(trajs, fams, hops, res["start"], 8)
+    achieved = algo / (bt_k / 1e3) / 1e9
+    info = {
+        "ms_per_step": ms, "total_hops": total_hops, "hops_per_s": value,
+        "waves_per_cu": wpc, "no_select": int(res["no_select"].sum()),
+        "roofline": {"bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "kmc_batched_kernel<double,true>",
+                     "launch_ms": bt_k, "algorithmic_bytes": algo},
+    }
+    bt.close()
+    return info, trajs, fams
+
+
+def run_plain(args, log):
+    from pycd_amd import workloads, _native, rng
+    from pycd_amd.rng import seeded_starts
+    w = workloads.corundum(8, n_carriers=16, field_vpm=1e6, t_final=2e-8)
+    table = w.build_table()
+    run = w.make_run(table)
+    pb = _native.KmcProblem(
+        table=table, lattice_q=run.base_charge_config()[:, 0],
+        nbr_count=run.nbr_count, nbr_site=run.nbr_site, nbr_vec=run.nbr_vec,
+        nbr_lambda=run.nbr_lambda, nbr_vab=run.nbr_vab,
+        site_energy=run.system_relative_energies,
+        carrier_q=np.asarray(run.species_charge_list, dtype=np.float64),
+        field=run.electric_field, nu=run.material.vn, kT=run.temp,
+        time_interval=run.time_interval,
+        n_path=int(run.t_final / run.time_interval) + 1)
+    sess = _native.KmcSession(pb)
+    occs, words = seeded_starts(2, 1, run.hop_site_lists["electron"], [16])
+    state = (3, tuple(int(v) for v in words[0]), None)
+    n = args.plain_hops
+    uni = rng.UniformStream(state).take(2 * n)
+    dev = _native.DeviceArray.from_host(uni)
+    sess.begin(occs[0])
+    ms = sess.advance_device(dev, n)
+    hops = sess.read()["hops"]
+    sess.close()
+    dev.free()
+    return {"hops": hops, "ms": ms, "hops_per_s": hops / (ms / 1e3),
+            "n_sites": w.n_sites, "n_carriers": 16}
+
+
+def cpu_baseline(args, trajs, fams, log):
+    """NumPy oracle on a bounded sample of the batched workload (1 core)."""
+    import random
+    from oracle import kmc
+    from pycd_amd.rng import UniformStream
+    rng_ = np.random.default_rng(99)
+    sample = rng_.choice(len(trajs), size=min(8, len(trajs)), replace=False)
+    budget = args.cpu_seconds / len(sample)
+    done_hops, spent = 0, 0.0
+    for k in sample:
+        t = trajs[k]
+        fam = fams[t["family"]]
+        run = fam.run
+        pb = kmc.Problem(fam.table, fam.lattice_q, run.nbr_count,
+                         run.nbr_site, run.nbr_vec, run.nbr_lambda,
+                         run.nbr_vab, run.system_relative_energies,
+                         [-1.0] * t["n_carriers"], t["field"], fam.nu,
+                         t["kT"], t["time_interval"], t["n_path"])
+        gen = random.Random()
+        gen.seed(int(k))
+        occ = gen.sample(fam.hop_sites, t["n_carriers"])
+        # time a fixed number of hops sized to the per-trajectory budget
+        probe = 20
+        t0 = time.perf_counter()
+        kmc.run(pb, occ, gen.random, max_hops=probe)
+        per = (time.perf_counter() - t0) / probe
+        n = max(probe, int(budget / max(per, 1e-6)))
+        t0 = time.perf_counter()
+        res = kmc.run(pb, occ, gen.random, max_hops=n)
+        spent += time.perf_counter() - t0
+        done_hops += res["hops"]
+    return {"value": done_hops / spent, "unit": "hops/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{len(sample)} trajectories of the batched workload, "
+                      f"{done_hops} hops, {spent:.1f} s, NumPy oracle of "
+                      f"Run.do_kmc_steps, 1 thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              f"trajectories are independent, see DESIGN.md for sharding",
+              file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    info, trajs, fams = run_batched(args, log)
+    log(f"batched: {info['hops_per_s']:.4e} hops/s, {info['ms_per_step']:.1f}"
+        f" ms/step")
+    plain = None
+    if not args.no_plain:
+        plain = run_plain(args, log)
+        log(f"plain: {plain['hops_per_s']:.4e} hops/s")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, trajs, fams, log)
+        log(f"cpu: {cpu['value']:.1f} hops/s")
+    line = {
+        "metric": "KMC hops/s summed, batched vs plain on 1 GPU; p50/p99 "
+                  "job latency",
+        "value": info["hops_per_s"],
+        "unit": "hops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": info["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 rates, f64 table/prefix/clock",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched: {len(trajs)} trajectories, corundum "
+                        f"edge 4-8 (768-6144 sites), 1-64 electrons, "
+                        f"|E| 0-1e7 V/m, T 250-400 K, {args.hops} hops each",
+            "global_batch": len(trajs), "seq_len": args.hops,
+            "parallelism": "1 GPU, one wavefront per trajectory",
+            "waves_per_cu": info["waves_per_cu"],
+        },
+        "plain_hops_per_s": plain["hops_per_s"] if plain else None,
+        "batched_over_plain": (info["hops_per_s"] / plain["hops_per_s"]
+                               if plain else None),
+        "p50_latency_ms": None,
+        "p99_latency_ms": None,
+        "roofline": info["roofline"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

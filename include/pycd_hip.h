@@ -59,6 +59,13 @@ const char* pycd_last_error(const pycd_ctx* ctx);
 /* Symbol present so that loaders can check the ABI revision. */
 int pycd_abi_version(void);
 
+/* Device buffers for callers without their own allocator (benchmarks,
+ * ctypes users): allocate, free, and copy in any direction (synchronous on
+ * the context's stream).  No reference counterpart. */
+int pycd_dev_alloc(pycd_ctx* ctx, int64_t bytes, void** out);
+int pycd_dev_free(pycd_ctx* ctx, void* p);
+int pycd_memcpy(pycd_ctx* ctx, void* dst, const void* src, int64_t bytes);
+
 /* ---------------------------------------------------------------- Ewald -- */
 typedef struct {
   int64_t n_kvec;        /* half-space k vectors with |k|^2 < k_cut^2       */
@@ -153,6 +160,10 @@ int pycd_kmc_advance(pycd_kmc* s, const double* uniforms, int64_t n_pairs,
  *   drift      n_c x 3     sum of hop vector * rate (PyCD/core.py:2822-2825)
  *   occupancy  n_c         current sites
  *   scalars    [t, energy, start_path_index, hops]                          */
+/* Device time (ms, HIP events on the context's stream) of the kernel of
+ * the last pycd_kmc_advance call. */
+double pycd_kmc_last_ms(const pycd_kmc* s);
+
 int pycd_kmc_read(pycd_kmc* s, double* unwrapped, double* energy,
                   double* delg_0, double* drift, int64_t* occupancy,
                   double scalars[4]);
@@ -165,6 +176,79 @@ int pycd_kmc_read(pycd_kmc* s, double* unwrapped, double* energy,
  *   sd   n_traj x n_lags x n_c                                              */
 int pycd_msd(pycd_ctx* ctx, const double* pos, int64_t n_traj,
              int64_t n_path, int64_t n_carriers, int64_t n_lags, double* sd);
+
+/* --------------------------------------------------------- batched KMC -- */
+/* Many independent trajectories (the loop over trajectories of
+ * Run.do_kmc_steps, PyCD/core.py:2668, each running the loop body of
+ * PyCD/core.py:2787-2861) in one persistent launch.  A family is one lattice
+ * size: its table and hop slots are shared by all its trajectories.  Every
+ * trajectory draws its uniforms from its own CPython MT19937 state on the
+ * device (4 words per hop, as random.random() twice). */
+typedef struct {
+  int64_t n_sites;            /* N                                          */
+  int32_t z;                  /* hop slots per hopping site (1..7)          */
+  int32_t max_z;              /* row stride of the slot tables              */
+  const double* table;        /* N x N pair table                           */
+  const double* lattice_q;    /* N lattice charges                          */
+  const int32_t* nbr_site;    /* N x max_z                                  */
+  const double* nbr_vec;      /* N x max_z x 3                              */
+  const double* nbr_lambda;   /* N x max_z                                  */
+  const double* nbr_vab;      /* N x max_z                                  */
+  const double* site_energy;  /* N                                          */
+} pycd_kmc_family;
+
+typedef struct {
+  int32_t family;             /* index into the families                    */
+  int32_t n_carriers;         /* 1..64                                      */
+  double carrier_q;           /* charge of every carrier                    */
+  double field[3];            /* au                                         */
+  double kT;                  /* Hartree                                    */
+  double time_interval;       /* path-step width (au time)                  */
+  double ewald_neut;          /* background term of the energy              */
+  int64_t n_path;             /* path rows                                  */
+  int64_t max_hops;           /* stop after this many hops (<= 0: time)     */
+} pycd_kmc_traj;
+
+typedef struct {
+  double t;                   /* clock after the last hop                   */
+  double energy;              /* energy after the last hop                  */
+  int64_t hops;
+  int64_t start;              /* path rows written: [0, start)              */
+  int32_t no_select;          /* hops where rounding left no cum > u1       */
+  int32_t done;
+} pycd_kmc_traj_result;
+
+typedef struct pycd_batch pycd_batch;
+
+/* Uploads the families; table_precision / rate_precision are PYCD_PREC_F64
+ * or PYCD_PREC_F32 (the prefix sum, selection and clock are FP64 always). */
+int pycd_batch_create(pycd_ctx* ctx, const pycd_kmc_family* fams,
+                      int32_t n_fam, double nu, int32_t table_precision,
+                      int32_t rate_precision, pycd_batch** out);
+void pycd_batch_destroy(pycd_batch* b);
+
+/* Trajectories, their concatenated initial occupancies, their MT states
+ * (n_traj x 625 words: the 624-word key and the position, as in
+ * random.getstate()), an optional launch order (a permutation; NULL: as
+ * given), optional per-trajectory MSD lag counts (NULL: no MSD) with the
+ * length scale applied to positions first, and waves per CU (<= 0: 8). */
+int pycd_batch_setup(pycd_batch* b, const pycd_kmc_traj* trajs,
+                     int64_t n_traj, const int64_t* occupancy,
+                     const uint32_t* mt_states, const int32_t* order,
+                     const int64_t* msd_lags, double msd_scale,
+                     int32_t waves_per_cu);
+
+/* Runs every trajectory (and the MSD reduction), optionally through a
+ * captured hipGraph; *elapsed_ms = device time between events around it. */
+int pycd_batch_run(pycd_batch* b, int32_t use_graph, double* elapsed_ms);
+
+/* Per-trajectory results and drift sums (n_traj x 64 x 3, carrier-major). */
+int pycd_batch_results(pycd_batch* b, pycd_kmc_traj_result* results,
+                       double* drift);
+/* Path rows of one trajectory (n_path x 3 n_c; rows never written are 0). */
+int pycd_batch_rows(pycd_batch* b, int64_t traj, double* rows);
+/* Carrier-averaged MSD per trajectory, concatenated (sum of lag counts). */
+int pycd_batch_msd(pycd_batch* b, double* msd);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,29 @@ class KmcProblemC(C.Structure):
                 ("n_path", C.c_int64)]
 
 
+class FamilyC(C.Structure):
+    _fields_ = [("n_sites", C.c_int64), ("z", C.c_int32),
+                ("max_z", C.c_int32), ("table", C.c_void_p),
+                ("lattice_q", C.c_void_p), ("nbr_site", C.c_void_p),
+                ("nbr_vec", C.c_void_p), ("nbr_lambda", C.c_void_p),
+                ("nbr_vab", C.c_void_p), ("site_energy", C.c_void_p)]
+
+
+class TrajC(C.Structure):
+    _fields_ = [("family", C.c_int32), ("n_carriers", C.c_int32),
+                ("carrier_q", C.c_double), ("field", C.c_double * 3),
+                ("kT", C.c_double), ("time_interval", C.c_double),
+                ("ewald_neut", C.c_double), ("n_path", C.c_int64),
+                ("max_hops", C.c_int64)]
+
+
+class TrajResultC(C.Structure):
+    _fields_ = [("t", C.c_double), ("energy", C.c_double),
+                ("hops", C.c_int64), ("start", C.c_int64),
+                ("no_select", C.c_int32), ("done", C.c_int32)]
+
+
+
 _P = C.c_void_p
 _SIGNATURES = {
     "pycd_abi_version": (C.c_int, []),
@@ -60,6 +83,23 @@ _SIGNATURES = {
     "pycd_kmc_read": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "pycd_msd": (C.c_int, [_P, _P, C.c_int64, C.c_int64, C.c_int64,
                            C.c_int64, _P]),
+    "pycd_batch_create": (C.c_int, [C.c_void_p, C.POINTER(FamilyC), C.c_int32,
+                                    C.c_double, C.c_int32, C.c_int32,
+                                    C.POINTER(C.c_void_p)]),
+    "pycd_batch_destroy": (None, [C.c_void_p]),
+    "pycd_batch_setup": (C.c_int, [C.c_void_p, C.POINTER(TrajC), C.c_int64,
+                                   C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_double, C.c_int32]),
+    "pycd_batch_run": (C.c_int, [C.c_void_p, C.c_int32,
+                                 C.POINTER(C.c_double)]),
+    "pycd_batch_results": (C.c_int, [C.c_void_p, C.POINTER(TrajResultC),
+                                     C.c_void_p]),
+    "pycd_batch_rows": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+    "pycd_batch_msd": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pycd_dev_alloc": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_void_p)]),
+    "pycd_dev_free": (C.c_int, [_P, _P]),
+    "pycd_memcpy": (C.c_int, [_P, _P, _P, C.c_int64]),
+    "pycd_kmc_last_ms": (C.c_double, [_P]),
 }
 
 
@@ -146,6 +186,39 @@ def pair_table(pos, cell, trans, pbc, alpha, r_cut, k_cut, eps_r,
 def table_energy_parts(report, q):
     raise NotImplementedError(
         "compute_energy_contributions is not part of this implementation")
+
+
+class DeviceArray:
+    """A device buffer of the context's device (bytes + dtype + shape)."""
+
+    def __init__(self, nbytes, dtype, shape):
+        self.ctx = context()
+        h = C.c_void_p()
+        _check(self.ctx, lib().pycd_dev_alloc(self.ctx, int(nbytes),
+                                              C.byref(h)), "pycd_dev_alloc")
+        self.ptr = h
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.shape = tuple(shape)
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes, a.dtype, a.shape)
+        _check(d.ctx, lib().pycd_memcpy(d.ctx, d.ptr, _ptr(a), a.nbytes),
+               "pycd_memcpy")
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(self.ctx, lib().pycd_memcpy(self.ctx, _ptr(out), self.ptr,
+                                           self.nbytes), "pycd_memcpy")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().pycd_dev_free(self.ctx, self.ptr)
+            self.ptr = None
 
 
 # -------------------------------------------------------------------- KMC --
@@ -241,6 +314,17 @@ class KmcSession:
         _check(self.ctx, rc, "pycd_kmc_advance")
         k = made.value
         return rc, times[:k], (procs[:k] if record else None)
+
+    def advance_device(self, dev_uniforms, n_pairs, max_hops=None):
+        """Advance from uniforms already in device memory; returns the
+        kernel's device time in ms."""
+        made = C.c_int64(0)
+        rc = lib().pycd_kmc_advance(self.h, dev_uniforms.ptr, int(n_pairs),
+                                    int(max_hops or 0), None, None,
+                                    C.byref(made))
+        _check(self.ctx, rc, "pycd_kmc_advance")
+        self.last_made = made.value
+        return lib().pycd_kmc_last_ms(self.h)
 
     def read(self):
         pb = self.pb

@@ -40,3 +40,28 @@ extern "C" void pycd_ctx_destroy(pycd_ctx* ctx) {
 extern "C" const char* pycd_last_error(const pycd_ctx* ctx) {
   return ctx ? ctx->error.c_str() : "null context";
 }
+
+extern "C" int pycd_dev_alloc(pycd_ctx* ctx, int64_t bytes, void** out) {
+  if (!ctx || !out || bytes <= 0) return PYCD_ERR_ARG;
+  pycd::DeviceGuard g(ctx->device);
+  *out = nullptr;
+  PYCD_HIP(ctx, hipMalloc(out, size_t(bytes)));
+  return PYCD_OK;
+}
+
+extern "C" int pycd_dev_free(pycd_ctx* ctx, void* p) {
+  if (!ctx) return PYCD_ERR_ARG;
+  pycd::DeviceGuard g(ctx->device);
+  if (p) PYCD_HIP(ctx, hipFree(p));
+  return PYCD_OK;
+}
+
+extern "C" int pycd_memcpy(pycd_ctx* ctx, void* dst, const void* src,
+                           int64_t bytes) {
+  if (!ctx || !dst || !src || bytes < 0) return PYCD_ERR_ARG;
+  pycd::DeviceGuard g(ctx->device);
+  PYCD_HIP(ctx, hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDefault,
+                               ctx->stream));
+  PYCD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PYCD_OK;
+}

@@ -1,0 +1,562 @@
+// Batched KMC: thousands of independent Run.do_kmc_steps trajectories
+// (PyCD/core.py:2668-2861) of uneven supercell size and carrier count in
+// one persistent launch on gfx950.
+//
+// Scheduling: one wavefront (64 lanes, one workgroup) owns one trajectory at
+// a time and takes the next from a global atomic queue when it finishes, in
+// the order given by the host (longest expected work first).  The queue has
+// an exit condition every wave reaches (index >= n_traj), so the grid drains.
+//
+// Per-hop algebra is the plain path's (q.A[x,:] = phi_L[x] + psi[x]), but
+// psi is maintained incrementally: when carrier m hops s -> n, every other
+// carrier's slot values change by q_m (A[n, x] - A[s, x]) (two gathers per
+// slot) and carrier m's own z+1 slots are recomputed from scratch
+// (n_c gathers each).  That is 3 (1 + z) n_c table reads per hop.
+//
+// Random numbers: each trajectory carries its CPython MT19937 state (624
+// words + position) in LDS and draws 4 words per hop exactly like
+// random.random() (u1 selects, u2 advances the clock), so a batched
+// trajectory consumes the same stream as the plain path would.
+//
+// Rates: FP64 (bit-faithful mode) or FP32 (throughput mode); the prefix
+// sum of k / k_tot, the selection and the clock are FP64 in both.
+#include "common.hpp"
+#include "mt19937.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace pycd {
+
+struct SlotRec {      // one hop slot of one site; 64 bytes
+  int32_t nbr;        // target site, -1 if the slot does not exist
+  int32_t pad;
+  double lam;         // lambda (Hartree)
+  double vab;         // V_ab (Hartree)
+  double tself;       // A[s,s] - A[s,n]
+  double vec[3];      // hop vector (bohr)
+  double pad2;
+};
+static_assert(sizeof(SlotRec) == 64, "slot record is one 64-byte line");
+
+struct FamDev {
+  const double* A64;
+  const float* A32;
+  const double* phiL;
+  const double* E;
+  const SlotRec* slots;
+  int64_t N;
+  int32_t z, mz;
+  double qLAqL;
+};
+
+struct TrajDev {
+  int32_t fam, nc;
+  double q;           // carrier charge
+  double field[3];    // au
+  double kT;          // Hartree
+  double dt;          // path-step width (au time)
+  double ewald_neut;
+  int64_t P;          // path rows
+  int64_t max_hops;   // <= 0: run until the path is full
+  int64_t row_off;    // offset (doubles) of this trajectory's path rows
+  int64_t occ_off;    // offset of its initial occupancy
+};
+
+struct TrajOut {
+  double t;
+  double energy;
+  int64_t hops;
+  int64_t start;
+  int32_t no_select;
+  int32_t done;
+  double pad[2];
+};
+
+struct BatchArgs {
+  const FamDev* fams;
+  const TrajDev* trajs;
+  const int32_t* order;
+  const int64_t* occ0;
+  const uint32_t* mt0;      // n_traj x 625
+  double* rows;             // all path rows
+  double* drift;            // n_traj x 64 x 3 (nc <= kMaxCarriers)
+  TrajOut* out;
+  uint32_t* counter;
+  int32_t n_traj;
+  int32_t nc_max;
+  int32_t z_max;
+  double nu;
+};
+
+constexpr int kMaxCarriers = 64;
+constexpr int kMaxPer = 8;    // processes per lane: n_c z <= 512
+constexpr double kEulerE = 2.718281828459045;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline double wave_excl_scan(double v, int lane) {
+  double incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double o = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += o;
+  }
+  const double excl = __shfl_up(incl, 1, 64);
+  return lane == 0 ? 0.0 : excl;
+}
+
+template <typename TT>
+__device__ inline double tab(const FamDev& f, int64_t idx);
+template <>
+__device__ inline double tab<double>(const FamDev& f, int64_t idx) {
+  return f.A64[idx];
+}
+template <>
+__device__ inline double tab<float>(const FamDev& f, int64_t idx) {
+  return double(f.A32[idx]);
+}
+
+template <typename TT, bool kF32Rate>
+__global__ void __launch_bounds__(64)
+kmc_batched_kernel(BatchArgs a) {
+  extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x;
+  const int rmax = a.nc_max * (1 + a.z_max);
+  uint32_t* mt = reinterpret_cast<uint32_t*>(smem_raw);           // 624
+  double* psi = reinterpret_cast<double*>(smem_raw + 2560);        // rmax
+  double* disp = psi + rmax;                                        // 3 nc
+  double* base = disp + 3 * a.nc_max;                               // 3 nc
+  double* drift = base + 3 * a.nc_max;                              // 3 nc
+  int32_t* sites = reinterpret_cast<int32_t*>(drift + 3 * a.nc_max);  // rmax
+  int32_t* occ = sites + rmax;                                      // nc
+
+  for (;;) {
+    uint32_t ti = 0;
+    if (lane == 0) ti = atomicAdd(a.counter, 1u);
+    ti = __shfl(ti, 0, 64);
+    if (ti >= uint32_t(a.n_traj)) break;
+    const int tr = a.order[ti];
+    const TrajDev td = a.trajs[tr];
+    const FamDev f = a.fams[td.fam];
+    const int nc = td.nc, z = f.z, zp = z + 1;
+    const int R = nc * zp, nproc = nc * z;
+    const int per = (nproc + 63) / 64;
+    const int64_t N = f.N;
+    const double q = td.q;
+    double* rows = a.rows + td.row_off;
+    const int w = 3 * nc;
+
+    // ---- load state
+    const uint32_t* m0 = a.mt0 + int64_t(tr) * 625;
+    for (int k = lane; k < 624; k += 64) mt[k] = m0[k];
+    int mti = int(m0[624]);
+    for (int c = lane; c < nc; c += 64) occ[c] = int32_t(a.occ0[td.occ_off + c]);
+    for (int k = lane; k < w; k += 64) {
+      disp[k] = 0.0;
+      base[k] = 0.0;
+      drift[k] = 0.0;
+      rows[k] = 0.0;  // path row 0: the initial positions
+    }
+    __syncthreads();
+    for (int r = lane; r < R; r += 64) {
+      const int c = r / zp, k = r - c * zp;
+      const int s = occ[c];
+      sites[r] = k == 0 ? s : f.slots[int64_t(s) * f.mz + k - 1].nbr;
+    }
+    __syncthreads();
+    for (int r = lane; r < R; r += 64) {
+      const int64_t row = int64_t(sites[r]) * N;
+      double acc = 0.0;
+      for (int c2 = 0; c2 < nc; ++c2) acc = fma(q, tab<TT>(f, row + occ[c2]), acc);
+      psi[r] = acc;
+    }
+    // initial energy: ewald_neut + q_L A q_L + sum_c q (2 phi_L + psi_self)
+    double e_loc = 0.0;
+    for (int c = lane; c < nc; c += 64) {
+      const int64_t row = int64_t(occ[c]) * N;
+      double inner = 0.0;
+      for (int c2 = 0; c2 < nc; ++c2) inner = fma(q, tab<TT>(f, row + occ[c2]), inner);
+      e_loc += q * (2.0 * f.phiL[occ[c]] + inner);
+    }
+    double energy = td.ewald_neut + (f.qLAqL + wave_sum(e_loc));
+    __syncthreads();
+
+    double t = 0.0;
+    int64_t start = 1, hops = 0;
+    int no_select = 0;
+    const bool field_on =
+        td.field[0] != 0.0 || td.field[1] != 0.0 || td.field[2] != 0.0;
+    const float kTf = float(td.kT);
+    const float nuf = float(a.nu);
+
+    for (;;) {
+      if (td.max_hops > 0 && hops >= td.max_hops) break;
+      // ---- rates of this lane's contiguous chunk of processes
+      double kv[kMaxPer], gv[kMaxPer];
+      double ksum = 0.0;
+#pragma unroll
+      for (int i = 0; i < kMaxPer; ++i) {
+        kv[i] = 0.0;
+        gv[i] = 0.0;
+        const int p = lane * per + i;
+        if (i < per && p < nproc) {
+          const int c = p / z;
+          const int j = p - c * z;
+          const int s = sites[c * zp];
+          const int n = sites[c * zp + 1 + j];
+          const SlotRec& rec = f.slots[int64_t(s) * f.mz + j];
+          const double term01 = (f.phiL[n] - f.phiL[s]) +
+                                (psi[c * zp + 1 + j] - psi[c * zp]);
+          const double g0 = 2 * q * (term01 + q * rec.tself) + (f.E[n] - f.E[s]);
+          double k;
+          if (kF32Rate) {
+            float shift = 0.f;
+            if (field_on)
+              shift = 0.5f * float(td.field[0] * rec.vec[0] +
+                                   td.field[1] * rec.vec[1] +
+                                   td.field[2] * rec.vec[2]);
+            const float lam = float(rec.lam);
+            const float gf = float(g0);
+            const float gs = ((lam + gf) * (lam + gf) / (4.f * lam) -
+                              float(rec.vab)) - shift;
+            k = double(nuf * __expf(-gs / kTf));
+          } else {
+            double shift = 0.0;
+            if (field_on)
+              shift = 0.5 * (td.field[0] * rec.vec[0] + td.field[1] * rec.vec[1] +
+                             td.field[2] * rec.vec[2]);
+            const double lam = rec.lam;
+            const double gs =
+                (((lam + g0) * (lam + g0) / (4 * lam)) - rec.vab) - shift;
+            k = a.nu * pow(kEulerE, -gs / td.kT);
+          }
+          kv[i] = k;
+          gv[i] = g0;
+          ksum += k;
+        }
+      }
+      const double k_tot = wave_sum(ksum);
+      // ---- two uniforms, as random.random() draws them
+      const uint32_t w0 = mt_next(mt, mti, lane);
+      const uint32_t w1 = mt_next(mt, mti, lane);
+      const uint32_t w2 = mt_next(mt, mti, lane);
+      const uint32_t w3 = mt_next(mt, mti, lane);
+      const double u1 = mt_uniform(w0, w1);
+      const double u2 = mt_uniform(w2, w3);
+      // ---- FP64 prefix of k / k_tot and selection
+      double xs = 0.0;
+#pragma unroll
+      for (int i = 0; i < kMaxPer; ++i) xs += kv[i] / k_tot;
+      double cum = wave_excl_scan(xs, lane);
+      int found = -1;
+      double g_found = 0.0, k_found = 0.0;
+#pragma unroll
+      for (int i = 0; i < kMaxPer; ++i) {
+        const int p = lane * per + i;
+        if (i < per && p < nproc) {
+          cum += kv[i] / k_tot;
+          if (found < 0 && cum > u1) {
+            found = p;
+            g_found = gv[i];
+            k_found = kv[i];
+          }
+        }
+      }
+      const unsigned long long mask = __ballot(found >= 0);
+      int src;
+      if (mask) {
+        src = __ffsll(static_cast<long long>(mask)) - 1;
+      } else {
+        // rounding left every cum <= u1: take the last process
+        src = (nproc - 1) / per;
+        if (lane == src) {
+          found = nproc - 1;
+          const int i = nproc - 1 - lane * per;
+#pragma unroll
+          for (int ii = 0; ii < kMaxPer; ++ii)
+            if (ii == i) {
+              g_found = gv[ii];
+              k_found = kv[ii];
+            }
+        }
+        ++no_select;
+      }
+      const int sel = __shfl(found, src, 64);
+      const double g = __shfl(g_found, src, 64);
+      const double ksel = __shfl(k_found, src, 64);
+      // ---- clock
+      int64_t end;
+      if (u2 > 0.0) {
+        t -= log(u2) / k_tot;
+        const double qq = t / td.dt;
+        end = qq < 9.0e18 ? int64_t(qq) : td.P;
+      } else {
+        t = INFINITY;
+        end = td.P;
+      }
+      // ---- state update
+      const int m = sel / z;
+      const int j = sel - m * z;
+      const int s_old = sites[m * zp];
+      const int s_new = sites[m * zp + 1 + j];
+      const SlotRec& rs = f.slots[int64_t(s_old) * f.mz + j];
+      energy += g;
+      if (lane < 3) {
+        disp[3 * m + lane] += rs.vec[lane];
+        if (field_on) drift[3 * m + lane] += rs.vec[lane] * ksel;
+      }
+      // psi of every other carrier's slots: += q (A[n, x] - A[s, x])
+      const int64_t rn = int64_t(s_new) * N, rso = int64_t(s_old) * N;
+      for (int r = lane; r < R; r += 64) {
+        const int c = r / zp;
+        if (c == m) continue;
+        const int x = sites[r];
+        psi[r] += q * (tab<TT>(f, rn + x) - tab<TT>(f, rso + x));
+      }
+      __syncthreads();
+      if (lane == 0) occ[m] = s_new;
+      if (lane < zp)
+        sites[m * zp + lane] =
+            lane == 0 ? s_new : f.slots[int64_t(s_new) * f.mz + lane - 1].nbr;
+      __syncthreads();
+      // carrier m's own slots from scratch: lanes over carriers c2
+      {
+        double part[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) part[k] = 0.0;
+        for (int c2 = lane; c2 < nc; c2 += 64) {
+          const int o = occ[c2];
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (k < zp) part[k] = fma(q, tab<TT>(f, int64_t(sites[m * zp + k]) * N + o), part[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (k < zp) {
+            const double v = wave_sum(part[k]);
+            if (lane == 0) psi[m * zp + k] = v;
+          }
+        }
+      }
+      // ---- path rows [start, end)
+      ++hops;
+      bool full = false;
+      if (end >= start + 1) {
+        if (end >= td.P) end = td.P;
+        __syncthreads();
+        for (int64_t idx = lane; idx < (end - start) * w; idx += 64) {
+          const int64_t r = start + idx / w;
+          const int k = int(idx % w);
+          rows[r * w + k] = base[k] + disp[k];
+        }
+        __syncthreads();
+        for (int k = lane; k < w; k += 64) {
+          base[k] += disp[k];
+          disp[k] = 0.0;
+        }
+        start = end;
+        full = end >= td.P;
+      }
+      __syncthreads();
+      if (full) break;
+    }
+    // ---- outputs
+    for (int k = lane; k < w; k += 64)
+      a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = drift[k];
+    if (lane == 0) {
+      TrajOut o;
+      o.t = t;
+      o.energy = energy;
+      o.hops = hops;
+      o.start = start;
+      o.no_select = no_select;
+      o.done = 1;
+      o.pad[0] = o.pad[1] = 0.0;
+      a.out[tr] = o;
+    }
+    __syncthreads();
+  }
+}
+
+// Squared displacement per trajectory, averaged over its carriers:
+//   msd[t, tau] = mean_c mean_{t0 < P_t - tau} |r_c(t0 + tau) - r_c(t0)|^2
+// One workgroup per (trajectory, block of 8 lags).
+struct MsdTraj {
+  int64_t row_off;
+  int64_t P;
+  int32_t nc;
+  int32_t pad;
+  int64_t out_off;
+  int64_t M;
+};
+
+__global__ void __launch_bounds__(256)
+msd_batched_kernel(const double* __restrict__ rows,
+                   const MsdTraj* __restrict__ tr,
+                   const TrajOut* __restrict__ outs, int n_traj, int nlb_max,
+                   double scale, double* __restrict__ out) {
+  const int t = blockIdx.x / nlb_max;
+  const int lb = blockIdx.x - t * nlb_max;
+  if (t >= n_traj) return;
+  MsdTraj d = tr[t];
+  // only the rows written by the run: [0, start) (all P when the path filled)
+  const int64_t written = outs[t].start;
+  const int64_t M_all = d.M;
+  d.P = written < 1 ? 1 : (written < d.P ? written : d.P);
+  const int64_t tau0 = int64_t(lb) * 8;
+  if (tau0 >= M_all) return;
+  const int w = 3 * d.nc;
+  const double* base = rows + d.row_off;
+  double acc[8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) acc[l] = 0.0;
+  // per lag the carriers are summed with the 1/(P - tau) weight applied at
+  // the end (same for every carrier of the trajectory)
+  for (int64_t idx = threadIdx.x; idx < d.P * d.nc; idx += blockDim.x) {
+    const int64_t t0 = idx / d.nc;
+    const int c = int(idx - t0 * d.nc);
+    const double* p0 = base + t0 * w + 3 * c;
+    const double x0 = p0[0] * scale, y0 = p0[1] * scale, z0 = p0[2] * scale;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const int64_t tau = tau0 + l;
+      if (tau >= 1 && tau < d.M && t0 + tau < d.P) {
+        const double* p1 = base + (t0 + tau) * w + 3 * c;
+        const double dx = p1[0] * scale - x0, dy = p1[1] * scale - y0,
+                     dz = p1[2] * scale - z0;
+        acc[l] += dx * dx + dy * dy + dz * dz;
+      }
+    }
+  }
+  __shared__ double part[8][256];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) part[l][threadIdx.x] = acc[l];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int l = 0; l < 8; ++l) part[l][threadIdx.x] += part[l][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) {
+    const int64_t tau = tau0 + threadIdx.x;
+    if (tau < M_all) {
+      double v = 0.0;
+      if (tau >= 1 && tau < d.M && tau < d.P)
+        v = part[threadIdx.x][0] / double(d.P - tau) / double(d.nc);
+      out[d.out_off + tau] = v;
+    }
+  }
+}
+
+__global__ void fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void fill_f64(double* p, int64_t n, double v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// Family preparation kernels.
+__global__ void __launch_bounds__(256)
+row_dot_f64(const double* __restrict__ A, const double* __restrict__ q,
+            int64_t N, double* __restrict__ out) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t i = int64_t(blockIdx.x) * 4 + wave; i < N;
+       i += int64_t(gridDim.x) * 4) {
+    const double* row = A + i * N;
+    double acc = 0.0;
+    for (int64_t j = lane; j < N; j += 64) acc = fma(row[j], q[j], acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) out[i] = acc;
+  }
+}
+
+__global__ void to_f32(const double* __restrict__ a, float* __restrict__ b,
+                       int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = float(a[i]);
+}
+
+__global__ void slot_self_terms(const double* __restrict__ A, int64_t N,
+                                SlotRec* __restrict__ slots, int mz) {
+  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= N * mz) return;
+  const int64_t s = idx / mz;
+  const int n = slots[idx].nbr;
+  slots[idx].tself = n >= 0 ? A[s * N + s] - A[s * N + n] : 0.0;
+}
+
+__global__ void dot_f64(const double* __restrict__ a,
+                        const double* __restrict__ b, int64_t n,
+                        double* __restrict__ out) {
+  __shared__ double part[256];
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) acc = fma(a[k], b[k], acc);
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
+}  // namespace pycd
+
+// ------------------------------------------------------------------- host --
+using namespace pycd;
+
+struct BatchFamily {
+  DevBuf<double> A64, phiL, E, qL, tmp;
+  DevBuf<float> A32;
+  DevBuf<SlotRec> slots;
+  int64_t N = 0;
+  int z = 0, mz = 0;
+  double qLAqL = 0.0;
+  std::vector<int32_t> nbr_host;  // for host-side validation
+};
+
+struct pycd_batch {
+  pycd_ctx* ctx = nullptr;
+  std::vector<BatchFamily*> fams;
+  DevBuf<FamDev> d_fams;
+  int table_prec = 0;  // 0 FP64, 1 FP32
+  int rate_prec = 0;   // 0 FP64, 1 FP32
+  double nu = 0.0;
+  // per-run buffers
+  DevBuf<TrajDev> d_trajs;
+  DevBuf<int32_t> d_order;
+  DevBuf<int64_t> d_occ;
+  DevBuf<uint32_t> d_mt, d_counter;
+  DevBuf<double> d_rows, d_drift, d_msd;
+  DevBuf<TrajOut> d_out;
+  DevBuf<MsdTraj> d_msdtr;
+  std::vector<TrajDev> trajs;
+  std::vector<int64_t> msd_off;
+  int64_t n_traj = 0, rows_total = 0, msd_total = 0;
+  int nc_max = 0, z_max = 0;
+  int nlb_max = 0;
+  double msd_scale = 1.0;
+  size_t smem = 0;
+  int grid = 0;
+  hipGraphExec_t graph = nullptr;
+  float last_ms = 0.f;
+  ~pycd_batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    for (auto* f : fams) delete f;
+  }
+};
+
+#include "batch_api.inc"

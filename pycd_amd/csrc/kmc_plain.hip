@@ -377,6 +377,12 @@ struct pycd_kmc {
   double qLAqL = 0.0;
   size_t smem = 0;
   bool begun = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  ~pycd_kmc() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
 };
 
 namespace {
@@ -639,13 +645,24 @@ extern "C" int pycd_kmc_advance(pycd_kmc* s, const double* uniforms,
   d.epath = s->epath.ptr;
   d.dpath = s->dpath.ptr;
   d.sc = s->sc.ptr;
+  if (!s->ev0) {
+    PYCD_HIP(ctx, hipEventCreate(&s->ev0));
+    PYCD_HIP(ctx, hipEventCreate(&s->ev1));
+  }
+  PYCD_HIP(ctx, hipEventRecord(s->ev0, st));
   kmc_plain_kernel<<<1, kThreads, s->smem, st>>>(d, dU, n_pairs, max_hops, dT,
                                                  dP, s->status.ptr);
   PYCD_HIP(ctx, hipGetLastError());
+  PYCD_HIP(ctx, hipEventRecord(s->ev1, st));
   int32_t status[2] = {-1, 0};
   PYCD_HIP(ctx, hipMemcpyAsync(status, s->status.ptr, sizeof(status),
                                hipMemcpyDeviceToHost, st));
   PYCD_HIP(ctx, hipStreamSynchronize(st));
+  {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    s->last_ms = ms;
+  }
   const int64_t made = status[1];
   if (t_host && made > 0)
     PYCD_HIP(ctx, hipMemcpy(hop_time, dT, 8 * size_t(made),
@@ -689,4 +706,8 @@ extern "C" int pycd_kmc_read(pycd_kmc* s, double* unwrapped, double* energy,
     scalars[3] = double(sc.hops);
   }
   return PYCD_OK;
+}
+
+extern "C" double pycd_kmc_last_ms(const pycd_kmc* s) {
+  return s ? s->last_ms : -1.0;
 }

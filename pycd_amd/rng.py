@@ -71,3 +71,29 @@ class UniformStream:
         """(key as uint32[624], pos) of a stdlib state, for device MT."""
         st = _to_numpy_state(py_state)["state"]
         return st["key"], st["pos"]
+
+
+def seeded_starts(random_seed, n_traj, hop_sites, n_carriers):
+    """Per-trajectory (occupancy, MT words, position) for a batch.
+
+    Follows the reference's seeding (PyCD/core.py:2573-2580): seed the
+    stdlib generator with `random_seed`, draw one float per trajectory,
+    reseed with it; then the initial carriers are
+    `sample(hop_sites[t], n_carriers[t])` (PyCD/core.py:2527).  The state
+    after the sample is what the trajectory's hops consume.
+    """
+    import random as _random
+    gen = _random.Random()
+    gen.seed(random_seed)
+    floats = [gen.random() for _ in range(n_traj)]
+    occs, words = [], np.zeros((n_traj, _N + 1), dtype=np.uint32)
+    for t in range(n_traj):
+        gen.seed(floats[t])
+        sites = hop_sites[t] if isinstance(hop_sites, (list, tuple)) \
+            else hop_sites
+        occs.append([int(v) for v in gen.sample(list(sites),
+                                                int(n_carriers[t]))])
+        key, pos = UniformStream.mt_state_words(gen.getstate())
+        words[t, :_N] = key
+        words[t, _N] = pos
+    return occs, words

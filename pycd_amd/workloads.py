@@ -187,3 +187,53 @@ def corundum(size, **kw):
 
 def monoclinic(size, **kw):
     return Workload(monoclinic_params(), size, **kw)
+
+
+def batched_config(n_traj=4096, hops=100_000, seed=2, param_seed=1234,
+                   edges=(4, 5, 6, 7, 8), nc_range=(1, 64),
+                   field_max_vpm=1e7, t_range=(250.0, 400.0),
+                   path_rows=1000, msd_lags=500):
+    """The batched workload: n_traj trajectories of uneven size and load.
+
+    Edge n ~ U{edges}, carriers ~ U{nc_range}, |E| ~ U(0, field_max) V/m
+    along a random lattice direction (`ld` = 1: fractional direction turned
+    Cartesian by the lattice), T ~ U(t_range) K, `hops` hops each.  The
+    trajectory parameters come from a NumPy generator seeded `param_seed`;
+    the KMC streams follow the reference's seeding protocol from `seed`.
+    The path-step width is chosen per trajectory so that `hops` hops span
+    about `path_rows` rows (2 x path_rows rows are allocated).
+    """
+    from . import batch as B
+    from .rng import seeded_starts
+    gen = np.random.default_rng(param_seed)
+    fams, works = [], []
+    for n in edges:
+        w = corundum(n, m_kcut=6)
+        table = w.build_table()
+        run = w.make_run(table)
+        fams.append(B.Family(run, table))
+        works.append(w)
+    sizes = gen.integers(0, len(edges), size=n_traj)
+    ncs = gen.integers(nc_range[0], nc_range[1] + 1, size=n_traj)
+    mags = gen.uniform(0.0, field_max_vpm, size=n_traj)
+    temps = gen.uniform(t_range[0], t_range[1], size=n_traj)
+    dirs = gen.integers(-1, 2, size=(n_traj, 3))
+    for k in range(n_traj):
+        while not np.any(dirs[k]):
+            dirs[k] = gen.integers(-1, 2, size=3)
+    occs, words = seeded_starts(seed, n_traj,
+                                [fams[s].hop_sites for s in sizes], ncs)
+    trajs = []
+    for k in range(n_traj):
+        fam = fams[sizes[k]]
+        lat = fam.run.material.lattice_matrix
+        d = dirs[k] @ lat
+        field = mags[k] * constants.VPM2AUFIELD * d / np.linalg.norm(d)
+        kT = temps[k] * constants.K2AUTEMP
+        rate = ncs[k] * B.expected_rate_per_carrier(fam, kT, field)
+        dt = hops / rate / path_rows
+        trajs.append(dict(family=int(sizes[k]), n_carriers=int(ncs[k]),
+                          carrier_q=-1.0, field=field, kT=kT,
+                          time_interval=dt, ewald_neut=0.0,
+                          n_path=2 * path_rows + 1, max_hops=hops))
+    return fams, works, trajs, occs, words, [msd_lags] * n_traj

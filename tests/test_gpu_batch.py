@@ -1,0 +1,112 @@
+"""Batched persistent KMC kernel vs the oracle, the MSD oracle and itself."""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import kmc, msd as omsd, seeding
+from pycd_amd import batch as B, constants, rng, workloads
+from tests.systems import oracle_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _families(sizes, **kw):
+    fams, runs = [], []
+    for n in sizes:
+        w = workloads.corundum(n, **kw)
+        table = w.build_table()
+        run = w.make_run(table)
+        fams.append(B.Family(run, table))
+        runs.append((w, run, table))
+    return fams, runs
+
+
+def _mixed_batch(fams, runs, specs, seed, max_hops):
+    """specs: list of (family, n_carriers, field_vpm, T)."""
+    n_traj = len(specs)
+    sites = [fams[f].hop_sites for f, _, _, _ in specs]
+    counts = [nc for _, nc, _, _ in specs]
+    occs, words = rng.seeded_starts(seed, n_traj, sites, counts)
+    trajs = []
+    for (f, nc, fv, T) in specs:
+        field = np.array([0.0, 0.0, fv * constants.VPM2AUFIELD])
+        trajs.append(dict(family=f, n_carriers=nc, carrier_q=-1.0,
+                          field=field, kT=T * constants.K2AUTEMP,
+                          time_interval=runs[f][1].time_interval,
+                          n_path=int(runs[f][1].t_final
+                                     / runs[f][1].time_interval) + 1,
+                          max_hops=max_hops))
+    return trajs, occs, words
+
+
+def _oracle(runs, spec, traj, seed, t_index, n_traj, max_hops):
+    f, nc, fv, T = spec
+    w, run, table = runs[f]
+    pb = oracle_problem(run, table=table, field=np.asarray(traj["field"]))
+    pb.kT = traj["kT"]
+    pb.carrier_q = np.full(nc, -1.0)
+    state = seeding.trajectory_states(seed, n_traj)[t_index]
+    gen = random.Random()
+    gen.setstate(state)
+    occ = seeding.initial_occupancy(gen, run.hop_site_lists["electron"], nc)
+    return kmc.run(pb, occ, gen.random, max_hops=max_hops)
+
+
+def test_fp64_batch_matches_oracle_per_trajectory():
+    fams, runs = _families([3, 4], t_final=2e-8)
+    specs = [(0, 1, 0.0, 300.0), (1, 5, 1e6, 300.0), (0, 16, 5e6, 350.0),
+             (1, 3, 0.0, 250.0), (1, 16, 2e6, 400.0), (0, 2, 1e7, 280.0)]
+    seed, max_hops = 7, 1500
+    trajs, occs, words = _mixed_batch(fams, runs, specs, seed, max_hops)
+    bt = B.KmcBatch(fams, "f64", "f64")
+    order = np.argsort([-s[1] for s in specs], kind="stable")
+    bt.setup(trajs, occs, words, order=order, waves_per_cu=2)
+    bt.run()
+    res = bt.results()
+    for k, spec in enumerate(specs):
+        ora = _oracle(runs, spec, trajs[k], seed, k, len(specs), max_hops)
+        assert res["hops"][k] == ora["hops"] == max_hops
+        assert abs(res["t"][k] - ora["time_data"][-1]) <= \
+            1e-12 * ora["time_data"][-1], k
+        rows = bt.rows(k)
+        assert np.max(np.abs(rows - ora["unwrapped"])) < 1e-8, k
+        scale = max(1.0, abs(ora["energy_now"]))
+        assert abs(res["energy"][k] - ora["energy_now"]) < 1e-9 * scale, k
+        assert res["start"][k] == ora["start"]
+        nc = spec[1]
+        assert np.allclose(res["drift"][k, :nc], ora["drift"], rtol=1e-9,
+                           atol=1e-300)
+    bt.close()
+
+
+def test_msd_and_graph_replays():
+    fams, runs = _families([3], t_final=2e-10, time_interval=1e-13)
+    specs = [(0, nc, 0.0, 300.0) for nc in (1, 4, 9, 12)]
+    seed = 11
+    trajs, occs, words = _mixed_batch(fams, runs, specs, seed, 4000)
+    lags = [200, 150, 100, 64]
+    scale = 1 / constants.ANG2BOHR
+    bt = B.KmcBatch(fams, "f64", "f32")
+    bt.setup(trajs, occs, words, msd_lags=lags, msd_scale=scale,
+             waves_per_cu=1)
+    bt.run(use_graph=False)
+    eager = bt.results()
+    eager_msd = bt.msd()
+    for k in range(len(specs)):
+        rows = bt.rows(k)
+        P = int(eager["start"][k])
+        nc = specs[k][1]
+        pos = rows[:P].reshape(P, nc, 3) * scale
+        ref = omsd.squared_displacement(pos, 1, P, lags[k])[0].mean(axis=1)
+        assert np.allclose(eager_msd[k], ref, rtol=1e-11, atol=1e-300)
+    # graph replays with eager runs in between reproduce the eager results
+    for _ in range(3):
+        bt.run(use_graph=True)
+        rep = bt.results()
+        for key in ("t", "hops", "energy", "start"):
+            assert np.array_equal(rep[key], eager[key]), key
+        for a, b in zip(bt.msd(), eager_msd):
+            assert np.array_equal(a, b)
+        bt.run(use_graph=False)
+    bt.close()
