@@ -46,6 +46,13 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--traffic-profile",
+                    default=str(ROOT / "profiles" / "batched_kmc_4096traj.md"),
+                    help="profile summary holding the measured HBM bytes per "
+                         "launch of the batched kernel")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="bytes of a device-to-device copy issued first "
+                         "(counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -64,6 +71,17 @@ def algorithmic_bytes_batched(trajs, fams, hops, starts, table_bytes):
         per_hop = 3 * (1 + z) * nc * table_bytes + nc * z * (64 + 32)
         total += per_hop * hops[k] + 24.0 * nc * starts[k]
     return total
+
+
+def measured_traffic(path):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    import re
+    try:
+        text = Path(path).read_text()
+    except OSError:
+        return None
+    m = re.search(r"traffic per launch = .*? = ([0-9.e+]+) B", text)
+    return float(m.group(1)) if m else None
 
 
 def run_batched(args, log):
@@ -100,7 +118,8 @@ def run_batched(args, log):
         "waves_per_cu": wpc, "no_select": int(res["no_select"].sum()),
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.traffic_profile),
                      "kernel": "kmc_batched_kernel<double,true>",
                      "launch_ms": bt_k, "algorithmic_bytes": algo},
     }
@@ -159,15 +178,19 @@ def cpu_baseline(args, trajs, fams, log):
         gen = random.Random()
         gen.seed(int(k))
         occ = gen.sample(fam.hop_sites, t["n_carriers"])
-        # time a fixed number of hops sized to the per-trajectory budget
+        # steady-state hops only: the per-trajectory set-up (initial energy
+        # q^T A q) is timed separately with max_hops=0 and subtracted
+        t0 = time.perf_counter()
+        kmc.run(pb, occ, gen.random, max_hops=0)
+        setup = time.perf_counter() - t0
         probe = 20
         t0 = time.perf_counter()
         kmc.run(pb, occ, gen.random, max_hops=probe)
-        per = (time.perf_counter() - t0) / probe
-        n = max(probe, int(budget / max(per, 1e-6)))
+        per = max((time.perf_counter() - t0 - setup) / probe, 1e-6)
+        n = max(probe, int(budget / per))
         t0 = time.perf_counter()
         res = kmc.run(pb, occ, gen.random, max_hops=n)
-        spent += time.perf_counter() - t0
+        spent += max(time.perf_counter() - t0 - setup, 1e-9)
         done_hops += res["hops"]
     return {"value": done_hops / spent, "unit": "hops/s", "cores": 1,
             "kind": "port",
@@ -185,6 +208,16 @@ def main():
         sys.exit(2)
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.copy_calibration > 0:
+        from pycd_amd import _native
+        n = args.copy_calibration // 8
+        a = _native.DeviceArray(n * 8, np.float64, (n,))
+        b = _native.DeviceArray(n * 8, np.float64, (n,))
+        for _ in range(3):
+            _native.lib().pycd_memcpy(a.ctx, b.ptr, a.ptr, n * 8)
+        a.free()
+        b.free()
+        log(f"calibration: 3 device copies of {n * 8} bytes")
     info, trajs, fams = run_batched(args, log)
     log(f"batched: {info['hops_per_s']:.4e} hops/s, {info['ms_per_step']:.1f}"
         f" ms/step")
