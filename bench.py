@@ -146,7 +146,7 @@ def run_plain(args, log):
         time_interval=run.time_interval,
         n_path=int(run.t_final / run.time_interval) + 1)
     sess = _native.KmcSession(pb)
-    occs, words = seeded_starts(2, 1, run.hop_site_lists["electron"], [16])
+    occs, words = seeded_starts(2, 1, [run.hop_site_lists["electron"]], [16])
     state = (3, tuple(int(v) for v in words[0]), None)
     n = args.plain_hops
     uni = rng.UniformStream(state).take(2 * n)
@@ -235,7 +235,7 @@ def run_precision(args, log):
     ref = tables["f64"]
     scale = float(np.max(np.abs(ref)))
     run = w.make_run(ref)
-    occs, words = seeded_starts(2, 1, run.hop_site_lists["electron"], [32])
+    occs, words = seeded_starts(2, 1, [run.hop_site_lists["electron"]], [32])
     q = run.base_charge_config()[:, 0].copy()
     q[occs[0]] -= 1.0
     e_ref = float(q @ ref @ q)

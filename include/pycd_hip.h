@@ -101,6 +101,29 @@ int pycd_pair_table(pycd_ctx* ctx, const double* pos, int64_t n,
                     double k_cut, double eps_r, int32_t precision,
                     double* table_out, pycd_ewald_report* report);
 
+/* ------------------------------------------------------------ neighbours -- */
+/* N x N x 3 minimum-image vectors pos[j] - pos[i]
+ * (Neighbors.get_pairwise_min_image_vector_data, PyCD/core.py:571-594).
+ * `out` host or device. */
+int pycd_min_image_vectors(pycd_ctx* ctx, const double* pos, int64_t n,
+                           const double cell[9], const int32_t pbc[3],
+                           double* out);
+
+/* Hop neighbour lists (Neighbors.hop_neighbor_sites, PyCD/core.py:485-569,
+ * for all shells of generate_neighbor_list, :596-668, in one pass): for
+ * centre c and shell s, the positions k into `targets` with
+ * lo[s] < |mic(pos[targets[k]] - pos[centers[c]])| <= hi[s], ascending in k.
+ * Outputs (host memory): count[c, s], index[c, s, 0..count), vec[c, s, .., 3]
+ * with room for max_per_shell entries; if a shell holds more, the call fails
+ * and `count` still holds the true counts.  At most 16 shells. */
+int pycd_hop_neighbors(pycd_ctx* ctx, const double* pos, int64_t n,
+                       const int64_t* centers, int64_t n_centers,
+                       const int64_t* targets, int64_t n_targets,
+                       const double cell[9], const int32_t pbc[3],
+                       const double* lo, const double* hi, int32_t n_shells,
+                       int32_t max_per_shell, int32_t* count, int32_t* index,
+                       double* vec);
+
 /* ------------------------------------------------------------- plain KMC -- */
 /* Inputs of one trajectory family (one table, one set of hop tables).
  * Per-site hop slots follow the process order of get_process_attributes
@@ -241,6 +264,33 @@ int pycd_batch_setup(pycd_batch* b, const pycd_kmc_traj* trajs,
 /* Runs every trajectory (and the MSD reduction), optionally through a
  * captured hipGraph; *elapsed_ms = device time between events around it. */
 int pycd_batch_run(pycd_batch* b, int32_t use_graph, double* elapsed_ms);
+
+/* Adds a family to an existing batch (index returned), or frees one that
+ * no busy slot uses.  Used by the serving loop's table cache. */
+int pycd_batch_add_family(pycd_batch* b, const pycd_kmc_family* f,
+                          int32_t* index);
+int pycd_batch_drop_family(pycd_batch* b, int32_t index);
+
+/* Serving (continuous batching).  A pool of n_slots trajectory slots with a
+ * path-row arena of rows_capacity doubles; every slot starts idle.
+ * pycd_batch_admit fills an idle slot (its path rows at row_off in the
+ * arena, which must not overlap a busy slot's); pycd_batch_step advances
+ * every admitted, unfinished slot by at most slice_hops hops in one launch
+ * (state is saved on the device between steps, so a trajectory's result
+ * does not depend on how it is sliced or what else runs); a finished or
+ * cancelled slot is handed back with pycd_batch_release. */
+int pycd_batch_pool(pycd_batch* b, int64_t n_slots, int32_t nc_max,
+                    int32_t z_max, int64_t rows_capacity,
+                    int32_t waves_per_cu);
+int pycd_batch_admit(pycd_batch* b, int64_t slot, const pycd_kmc_traj* traj,
+                     const int64_t* occupancy, const uint32_t* mt_state,
+                     int64_t row_off);
+int pycd_batch_release(pycd_batch* b, int64_t slot);
+int pycd_batch_step(pycd_batch* b, int64_t slice_hops, double* elapsed_ms);
+/* Carrier-averaged MSD of the given pool slots (lags[k] lags each, positions
+ * scaled by `scale` first), concatenated. */
+int pycd_batch_slot_msd(pycd_batch* b, const int64_t* slots, int64_t n,
+                        const int64_t* lags, double scale, double* out);
 
 /* Per-trajectory results and drift sums (n_traj x 64 x 3, carrier-major). */
 int pycd_batch_results(pycd_batch* b, pycd_kmc_traj_result* results,

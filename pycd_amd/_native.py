@@ -74,6 +74,10 @@ _SIGNATURES = {
     "pycd_pair_table": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P, C.c_double,
                                   C.c_double, C.c_double, C.c_double,
                                   C.c_int32, _P, C.POINTER(EwaldReport)]),
+    "pycd_min_image_vectors": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P]),
+    "pycd_hop_neighbors": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, _P,
+                                     C.c_int64, _P, _P, _P, _P, C.c_int32,
+                                     C.c_int32, _P, _P, _P]),
     "pycd_kmc_create": (C.c_int, [_P, C.POINTER(KmcProblemC),
                                   C.POINTER(C.c_void_p)]),
     "pycd_kmc_destroy": (None, [_P]),
@@ -96,6 +100,17 @@ _SIGNATURES = {
                                      C.c_void_p]),
     "pycd_batch_rows": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
     "pycd_batch_msd": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pycd_batch_add_family": (C.c_int, [_P, C.POINTER(FamilyC),
+                                        C.POINTER(C.c_int32)]),
+    "pycd_batch_drop_family": (C.c_int, [_P, C.c_int32]),
+    "pycd_batch_pool": (C.c_int, [_P, C.c_int64, C.c_int32, C.c_int32,
+                                  C.c_int64, C.c_int32]),
+    "pycd_batch_admit": (C.c_int, [_P, C.c_int64, C.POINTER(TrajC), _P, _P,
+                                   C.c_int64]),
+    "pycd_batch_release": (C.c_int, [_P, C.c_int64]),
+    "pycd_batch_step": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_double)]),
+    "pycd_batch_slot_msd": (C.c_int, [_P, _P, C.c_int64, _P, C.c_double,
+                                      _P]),
     "pycd_dev_alloc": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_void_p)]),
     "pycd_dev_free": (C.c_int, [_P, _P]),
     "pycd_memcpy": (C.c_int, [_P, _P, _P, C.c_int64]),
@@ -219,6 +234,47 @@ class DeviceArray:
         if self.ptr:
             lib().pycd_dev_free(self.ctx, self.ptr)
             self.ptr = None
+
+
+# ------------------------------------------------------------- neighbours --
+def min_image_vectors(pos, cell, pbc):
+    pos = _f64(pos)
+    n = pos.shape[0]
+    out = np.empty((n, n, 3))
+    cell = _f64(cell, (9,))
+    pbc = np.ascontiguousarray(pbc, dtype=np.int32).reshape(3)
+    ctx = context()
+    _check(ctx, lib().pycd_min_image_vectors(ctx, _ptr(pos), n, _ptr(cell),
+                                             _ptr(pbc), _ptr(out)),
+           "pycd_min_image_vectors")
+    return out
+
+
+def hop_neighbors(pos, centers, targets, cell, pbc, lo, hi, max_per=64):
+    """Per centre and shell: (positions into targets, vectors)."""
+    pos = _f64(pos)
+    centers = np.ascontiguousarray(centers, dtype=np.int64)
+    targets = np.ascontiguousarray(targets, dtype=np.int64)
+    lo = _f64(lo)
+    hi = _f64(hi)
+    ns = len(lo)
+    cell = _f64(cell, (9,))
+    pbc = np.ascontiguousarray(pbc, dtype=np.int32).reshape(3)
+    ctx = context()
+    while True:
+        count = np.empty((len(centers), ns), dtype=np.int32)
+        index = np.empty((len(centers), ns, max_per), dtype=np.int32)
+        vec = np.empty((len(centers), ns, max_per, 3))
+        rc = lib().pycd_hop_neighbors(ctx, _ptr(pos), len(pos), _ptr(centers),
+                                      len(centers), _ptr(targets),
+                                      len(targets), _ptr(cell), _ptr(pbc),
+                                      _ptr(lo), _ptr(hi), ns, max_per,
+                                      _ptr(count), _ptr(index), _ptr(vec))
+        if rc < 0 and count.max() > max_per:
+            max_per = int(count.max())
+            continue
+        _check(ctx, rc, "pycd_hop_neighbors")
+        return count, index, vec
 
 
 # -------------------------------------------------------------------- KMC --

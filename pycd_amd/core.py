@@ -30,8 +30,7 @@ import numpy as np
 
 from . import constants
 from .io import read_poscar, generate_report
-from .lattice import (z_sorted_unit_cell, supercell_sites, MinimumImage,
-                      hop_window)
+from .lattice import z_sorted_unit_cell, supercell_sites, MinimumImage
 from . import rng
 from . import _native
 
@@ -208,32 +207,38 @@ class Neighbors(object):
     def hop_neighbor_sites(self, bulk_sites, center_site_indices,
                            neighbor_site_indices, cutoff_dist_limits,
                            cutoff_dist_key=None):
-        """Neighbours in the window (lo, hi] (PyCD/core.py:485-569)."""
-        idx, vecs = hop_window(bulk_sites.cell_coordinates,
-                               np.asarray(center_site_indices),
-                               np.asarray(neighbor_site_indices),
-                               cutoff_dist_limits[0], cutoff_dist_limits[1],
-                               self._mic)
-        sys_index = bulk_sites.system_element_index_list[
-            np.asarray(neighbor_site_indices)]
-        n = len(idx)
-        nbr = np.empty(n, dtype=object)
-        disp = np.empty(n, dtype=object)
-        for k in range(n):
-            nbr[k] = sys_index[idx[k]]
-            disp[k] = np.asarray(vecs[k])
-        counts = np.array([len(v) for v in idx], dtype=int)
-        return ReturnValues(neighbor_system_element_indices=nbr,
-                            displacement_vector_list=disp,
-                            num_neighbors=counts)
+        """Neighbours in the window (lo, hi] (PyCD/core.py:485-569), GPU."""
+        return self._hop_shells(bulk_sites, center_site_indices,
+                                neighbor_site_indices,
+                                [cutoff_dist_limits])[0]
+
+    def _hop_shells(self, bulk_sites, centers, targets, limits):
+        """All windows in one pass of the GPU neighbour kernel."""
+        centers = np.asarray(centers, dtype=np.int64)
+        targets = np.asarray(targets, dtype=np.int64)
+        lo = [float(a) for a, _ in limits]
+        hi = [float(b) for _, b in limits]
+        count, index, vec = _native.hop_neighbors(
+            bulk_sites.cell_coordinates, centers, targets,
+            self.simulation_cell_matrix, self.pbc, lo, hi)
+        sys_index = bulk_sites.system_element_index_list[targets]
+        out = []
+        for s in range(len(limits)):
+            nbr = np.empty(len(centers), dtype=object)
+            disp = np.empty(len(centers), dtype=object)
+            for k in range(len(centers)):
+                m = int(count[k, s])
+                nbr[k] = sys_index[index[k, s, :m]]
+                disp[k] = vec[k, s, :m].copy()
+            out.append(ReturnValues(neighbor_system_element_indices=nbr,
+                                    displacement_vector_list=disp,
+                                    num_neighbors=count[:, s].astype(int)))
+        return out
 
     def get_pairwise_min_image_vector_data(self, dst_path):
         """N x N x 3 minimum-image vectors to disk (PyCD/core.py:571-594)."""
-        pos = self.bulk_sites.cell_coordinates
-        n = len(pos)
-        out = np.zeros((n, n, 3))
-        for i in range(n):
-            out[i] = self._mic.apply(pos - pos[i])
+        out = _native.min_image_vectors(self.bulk_sites.cell_coordinates,
+                                        self.simulation_cell_matrix, self.pbc)
         np.save(Path(dst_path).joinpath("pairwise_min_image_vector_data.npy"),
                 out)
         return None
@@ -257,12 +262,10 @@ class Neighbors(object):
             sites = self.hopping_site_indices(t_index)
             classes = []
             for c, cutoffs in enumerate(per_class):
-                shells = []
-                for t, cut in enumerate(cutoffs):
-                    lim = [cut - tol[key][c][t], cut + tol[key][c][t]]
-                    shells.append(self.hop_neighbor_sites(
-                        self.bulk_sites, sites, sites, lim, key))
-                classes.append(shells)
+                limits = [[cut - tol[key][c][t], cut + tol[key][c][t]]
+                          for t, cut in enumerate(cutoffs)]
+                classes.append(self._hop_shells(self.bulk_sites, sites, sites,
+                                                limits))
             out[key] = classes
         return out
 

@@ -69,8 +69,10 @@ struct TrajOut {
   int64_t hops;
   int64_t start;
   int32_t no_select;
-  int32_t done;
-  double pad[2];
+  int32_t done;      // 1: the trajectory reached max_hops or filled its path
+  int32_t started;   // 1: state arrays hold a run in progress
+  int32_t pad0;
+  double pad1;
 };
 
 struct BatchArgs {
@@ -83,6 +85,12 @@ struct BatchArgs {
   double* drift;            // n_traj x 64 x 3 (nc <= kMaxCarriers)
   TrajOut* out;
   uint32_t* counter;
+  // resumable state (slices): MT words, occupancy, displacement/base/drift
+  uint32_t* mts;            // n_traj x 625
+  int32_t* st_occ;          // n_traj x kMaxCarriers
+  double* st_vec;           // n_traj x 3 x (3 kMaxCarriers)
+  double* st_psi;           // n_traj x kPsiStride
+  int64_t slice_hops;       // <= 0: run each trajectory to its end
   int32_t n_traj;
   int32_t nc_max;
   int32_t z_max;
@@ -91,6 +99,8 @@ struct BatchArgs {
 
 constexpr int kMaxCarriers = 64;
 constexpr int kMaxPer = 8;    // processes per lane: n_c z <= 512
+constexpr int kPsiStride = kMaxCarriers * 8;  // n_c (1 + z), z <= 7
+constexpr int kVecStride = 3 * 3 * kMaxCarriers;
 constexpr double kEulerE = 2.718281828459045;
 
 __device__ inline double wave_sum(double v) {
@@ -141,6 +151,9 @@ kmc_batched_kernel(BatchArgs a) {
     ti = __shfl(ti, 0, 64);
     if (ti >= uint32_t(a.n_traj)) break;
     const int tr = a.order[ti];
+    const TrajOut o0 = a.out[tr];
+    if (o0.done) continue;
+    const bool resume = o0.started != 0;
     const TrajDev td = a.trajs[tr];
     const FamDev f = a.fams[td.fam];
     const int nc = td.nc, z = f.z, zp = z + 1;
@@ -151,16 +164,27 @@ kmc_batched_kernel(BatchArgs a) {
     double* rows = a.rows + td.row_off;
     const int w = 3 * nc;
 
-    // ---- load state
-    const uint32_t* m0 = a.mt0 + int64_t(tr) * 625;
+    // ---- load state: fresh start, or the state saved by an earlier slice
+    uint32_t* mts = a.mts + int64_t(tr) * 625;
+    int32_t* socc = a.st_occ + int64_t(tr) * kMaxCarriers;
+    double* svec = a.st_vec + int64_t(tr) * kVecStride;
+    double* spsi = a.st_psi + int64_t(tr) * kPsiStride;
+    const uint32_t* m0 = resume ? mts : a.mt0 + int64_t(tr) * 625;
     for (int k = lane; k < 624; k += 64) mt[k] = m0[k];
     int mti = int(m0[624]);
-    for (int c = lane; c < nc; c += 64) occ[c] = int32_t(a.occ0[td.occ_off + c]);
+    for (int c = lane; c < nc; c += 64)
+      occ[c] = resume ? socc[c] : int32_t(a.occ0[td.occ_off + c]);
     for (int k = lane; k < w; k += 64) {
-      disp[k] = 0.0;
-      base[k] = 0.0;
-      drift[k] = 0.0;
-      rows[k] = 0.0;  // path row 0: the initial positions
+      if (resume) {
+        disp[k] = svec[k];
+        base[k] = svec[3 * kMaxCarriers + k];
+        drift[k] = svec[6 * kMaxCarriers + k];
+      } else {
+        disp[k] = 0.0;
+        base[k] = 0.0;
+        drift[k] = 0.0;
+        rows[k] = 0.0;  // path row 0: the initial positions
+      }
     }
     __syncthreads();
     for (int r = lane; r < R; r += 64) {
@@ -169,33 +193,54 @@ kmc_batched_kernel(BatchArgs a) {
       sites[r] = k == 0 ? s : f.slots[int64_t(s) * f.mz + k - 1].nbr;
     }
     __syncthreads();
-    for (int r = lane; r < R; r += 64) {
-      const int64_t row = int64_t(sites[r]) * N;
-      double acc = 0.0;
-      for (int c2 = 0; c2 < nc; ++c2) acc = fma(q, tab<TT>(f, row + occ[c2]), acc);
-      psi[r] = acc;
+    double t, energy;
+    int64_t start, hops;
+    int no_select;
+    if (resume) {
+      for (int r = lane; r < R; r += 64) psi[r] = spsi[r];
+      t = o0.t;
+      energy = o0.energy;
+      start = o0.start;
+      hops = o0.hops;
+      no_select = o0.no_select;
+    } else {
+      for (int r = lane; r < R; r += 64) {
+        const int64_t row = int64_t(sites[r]) * N;
+        double acc = 0.0;
+        for (int c2 = 0; c2 < nc; ++c2)
+          acc = fma(q, tab<TT>(f, row + occ[c2]), acc);
+        psi[r] = acc;
+      }
+      // initial energy: ewald_neut + q_L A q_L + sum_c q (2 phi_L + psi_self)
+      double e_loc = 0.0;
+      for (int c = lane; c < nc; c += 64) {
+        const int64_t row = int64_t(occ[c]) * N;
+        double inner = 0.0;
+        for (int c2 = 0; c2 < nc; ++c2)
+          inner = fma(q, tab<TT>(f, row + occ[c2]), inner);
+        e_loc += q * (2.0 * f.phiL[occ[c]] + inner);
+      }
+      energy = td.ewald_neut + (f.qLAqL + wave_sum(e_loc));
+      t = 0.0;
+      start = 1;
+      hops = 0;
+      no_select = 0;
     }
-    // initial energy: ewald_neut + q_L A q_L + sum_c q (2 phi_L + psi_self)
-    double e_loc = 0.0;
-    for (int c = lane; c < nc; c += 64) {
-      const int64_t row = int64_t(occ[c]) * N;
-      double inner = 0.0;
-      for (int c2 = 0; c2 < nc; ++c2) inner = fma(q, tab<TT>(f, row + occ[c2]), inner);
-      e_loc += q * (2.0 * f.phiL[occ[c]] + inner);
-    }
-    double energy = td.ewald_neut + (f.qLAqL + wave_sum(e_loc));
     __syncthreads();
-
-    double t = 0.0;
-    int64_t start = 1, hops = 0;
-    int no_select = 0;
+    int64_t slice_done = 0;
+    bool finished = false;
     const bool field_on =
         td.field[0] != 0.0 || td.field[1] != 0.0 || td.field[2] != 0.0;
     const float kTf = float(td.kT);
     const float nuf = float(a.nu);
 
     for (;;) {
-      if (td.max_hops > 0 && hops >= td.max_hops) break;
+      if (td.max_hops > 0 && hops >= td.max_hops) {
+        finished = true;
+        break;
+      }
+      if (a.slice_hops > 0 && slice_done >= a.slice_hops) break;
+      ++slice_done;
       // ---- rates of this lane's contiguous chunk of processes
       double kv[kMaxPer], gv[kMaxPer];
       double ksum = 0.0;
@@ -363,20 +408,33 @@ kmc_batched_kernel(BatchArgs a) {
         full = end >= td.P;
       }
       __syncthreads();
-      if (full) break;
+      if (full) {
+        finished = true;
+        break;
+      }
     }
-    // ---- outputs
-    for (int k = lane; k < w; k += 64)
+    // ---- outputs and resumable state
+    for (int k = lane; k < w; k += 64) {
       a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = drift[k];
+      svec[k] = disp[k];
+      svec[3 * kMaxCarriers + k] = base[k];
+      svec[6 * kMaxCarriers + k] = drift[k];
+    }
+    for (int k = lane; k < 624; k += 64) mts[k] = mt[k];
+    for (int c = lane; c < nc; c += 64) socc[c] = occ[c];
+    for (int r = lane; r < R; r += 64) spsi[r] = psi[r];
     if (lane == 0) {
+      mts[624] = uint32_t(mti);
       TrajOut o;
       o.t = t;
       o.energy = energy;
       o.hops = hops;
       o.start = start;
       o.no_select = no_select;
-      o.done = 1;
-      o.pad[0] = o.pad[1] = 0.0;
+      o.done = finished ? 1 : 0;
+      o.started = 1;
+      o.pad0 = 0;
+      o.pad1 = 0.0;
       a.out[tr] = o;
     }
     __syncthreads();
@@ -390,7 +448,7 @@ struct MsdTraj {
   int64_t row_off;
   int64_t P;
   int32_t nc;
-  int32_t pad;
+  int32_t traj;     // index into the trajectory results (rows written)
   int64_t out_off;
   int64_t M;
 };
@@ -405,7 +463,7 @@ msd_batched_kernel(const double* __restrict__ rows,
   if (t >= n_traj) return;
   MsdTraj d = tr[t];
   // only the rows written by the run: [0, start) (all P when the path filled)
-  const int64_t written = outs[t].start;
+  const int64_t written = outs[d.traj].start;
   const int64_t M_all = d.M;
   d.P = written < 1 ? 1 : (written < d.P ? written : d.P);
   const int64_t tau0 = int64_t(lb) * 8;
@@ -458,6 +516,14 @@ __global__ void fill_u32(uint32_t* p, int64_t n, uint32_t v) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
     p[i] = v;
+}
+
+__global__ void reset_outputs(TrajOut* o, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    o[i].done = 0;
+    o[i].started = 0;
+  }
 }
 
 __global__ void fill_f64(double* p, int64_t n, double v) {
@@ -542,6 +608,14 @@ struct pycd_batch {
   DevBuf<uint32_t> d_mt, d_counter;
   DevBuf<double> d_rows, d_drift, d_msd;
   DevBuf<TrajOut> d_out;
+  DevBuf<uint32_t> d_mts;
+  DevBuf<int32_t> d_socc;
+  DevBuf<double> d_svec, d_spsi;
+  // slot pool (serving)
+  bool pool = false;
+  int64_t rows_capacity = 0;
+  std::vector<char> slot_busy;
+  std::vector<MsdTraj> msd_host;
   DevBuf<MsdTraj> d_msdtr;
   std::vector<TrajDev> trajs;
   std::vector<int64_t> msd_off;

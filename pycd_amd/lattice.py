@@ -6,10 +6,10 @@ per-site hop tables that the HIP kernels consume.  It follows
 
   * Material.__init__ / generate_sites   PyCD/core.py:42-241
   * minimum image                       PyCD/core.py:292-361
-  * hop neighbour window                PyCD/core.py:485-569, 596-668
 
-The O(N^2) neighbour search is vectorised over centre blocks; the table
-itself (N^2 Ewald entries) is never built here.
+The O(N^2) neighbour search and minimum-image tables run on the GPU
+(pycd_hop_neighbors / pycd_min_image_vectors); the class below is the
+single-vector form used by the API's small helpers.
 """
 import numpy as np
 
@@ -104,25 +104,3 @@ class MinimumImage:
             best[better] = cand[better]
             best_n[better] = cand_n[better]
         return best.reshape(shape)
-
-
-def hop_window(coords, centers, targets, lo, hi, mic, block=256):
-    """Neighbours of each centre with lo < |r| <= hi (core.py:485-569).
-
-    Returns, per centre, the ascending target indices (positions into
-    `targets`) and their minimum-image vectors.
-    """
-    nbr_idx = []
-    nbr_vec = []
-    tpos = coords[targets]
-    for b0 in range(0, len(centers), block):
-        cpos = coords[centers[b0:b0 + block]]
-        d = tpos[np.newaxis, :, :] - cpos[:, np.newaxis, :]
-        v = mic.apply(d)
-        r = np.sqrt(np.einsum("ijk,ijk->ij", v, v))
-        hit = (lo < r) & (r <= hi)
-        for row in range(len(cpos)):
-            cols = np.nonzero(hit[row])[0]
-            nbr_idx.append(cols)
-            nbr_vec.append(v[row, cols])
-    return nbr_idx, nbr_vec

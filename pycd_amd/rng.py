@@ -76,6 +76,9 @@ class UniformStream:
 def seeded_starts(random_seed, n_traj, hop_sites, n_carriers):
     """Per-trajectory (occupancy, MT words, position) for a batch.
 
+    `hop_sites[t]` and `n_carriers[t]` are trajectory t's hopping sites (in
+    system order) and carrier count.
+
     Follows the reference's seeding (PyCD/core.py:2573-2580): seed the
     stdlib generator with `random_seed`, draw one float per trajectory,
     reseed with it; then the initial carriers are
@@ -89,9 +92,7 @@ def seeded_starts(random_seed, n_traj, hop_sites, n_carriers):
     occs, words = [], np.zeros((n_traj, _N + 1), dtype=np.uint32)
     for t in range(n_traj):
         gen.seed(floats[t])
-        sites = hop_sites[t] if isinstance(hop_sites, (list, tuple)) \
-            else hop_sites
-        occs.append([int(v) for v in gen.sample(list(sites),
+        occs.append([int(v) for v in gen.sample(list(hop_sites[t]),
                                                 int(n_carriers[t]))])
         key, pos = UniformStream.mt_state_words(gen.getstate())
         words[t, :_N] = key

@@ -14,7 +14,9 @@ def build_example(material):
     nb = Neighbors(mat, np.array(sim["system_size"]), np.array(sim["pbc"]))
     table = np.load(d / "precomputed_array.npy")
     pairwise = np.load(d / "pairwise_min_image_vector_data.npy")
-    system = System(mat, nb, None, pairwise, cfg["alpha"], cfg["r_cut"],
+    from oracle.neighbors import hop_neighbor_list
+    system = System(mat, nb, hop_neighbor_list(mat, nb), pairwise,
+                    cfg["alpha"], cfg["r_cut"],
                     cfg["k_cut"], cfg["precision_parameters"], [], [])
     run = Run(system, table, sim["temp"], sim["ion_charge_type"],
               sim["species_charge_type"], sim["n_traj"], sim["t_final"],

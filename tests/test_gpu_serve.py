@@ -1,0 +1,104 @@
+"""Serving loop: continuous batching, table cache, admission by footprint."""
+import numpy as np
+import pytest
+
+from pycd_amd import workloads
+from pycd_amd.serve import KmcServer, Job, RowArena, poisson_jobs
+
+pytestmark = pytest.mark.gpu
+
+_FAMILIES = {}
+
+
+def corundum_family(edge):
+    if edge not in _FAMILIES:
+        w = workloads.corundum(edge)
+        table = w.build_table()
+        _FAMILIES[edge] = (w.make_run(table), table)
+    return _FAMILIES[edge]
+
+
+def _result(job):
+    r = job.result
+    return (r["t"], r["energy"], r["hops"], r["start"], r["rows"].tobytes(),
+            r["msd"].tobytes())
+
+
+def test_same_alone_and_in_the_mix():
+    probe = dict(edge=3, n_carriers=6, hops=3000, seed=77, temp=320.0,
+                 field_vpm=2e6)
+    alone = KmcServer(corundum_family, n_slots=8, nc_max=16,
+                      rows_capacity=1 << 20, slice_hops=3000,
+                      rate_precision="f32", path_rows=50, msd_lags=20)
+    a = Job(0, **probe)
+    alone.submit(a)
+    alone.drain()
+    alone.close()
+    mixed = KmcServer(corundum_family, n_slots=64, nc_max=16,
+                      rows_capacity=1 << 22, slice_hops=257,
+                      rate_precision="f32", path_rows=50, msd_lags=20)
+    rng = np.random.default_rng(0)
+    jobs = [Job(k + 1, edge=int(rng.integers(3, 5)),
+                n_carriers=int(rng.integers(1, 17)),
+                hops=int(rng.integers(500, 4000)), seed=100 + k)
+            for k in range(40)]
+    b = Job(99, **probe)
+    for j in jobs[:20]:
+        mixed.submit(j)
+    mixed.submit(b)
+    for j in jobs[20:]:
+        mixed.submit(j)
+    mixed.drain()
+    mixed.close()
+    # a job ends at its hop budget or when its path rows are full
+    assert 0 < a.result["hops"] <= 3000
+    assert _result(a) == _result(b)
+    assert all(j.result is not None for j in jobs)
+
+
+def test_table_budget_forces_eviction():
+    """Tables of 324 and 768 sites (0.84 MB, 4.7 MB) do not fit a 5 MB
+    budget together: sizes alternate, so tables are evicted and rebuilt,
+    and every job still completes.  A job whose table alone exceeds the
+    budget is rejected instead of waiting forever."""
+    srv = KmcServer(corundum_family, n_slots=16, nc_max=8,
+                    rows_capacity=1 << 20, table_budget_bytes=5_000_000,
+                    slice_hops=500, path_rows=20, msd_lags=5)
+    jobs = [Job(k, edge=3 if k % 2 == 0 else 4, n_carriers=2, hops=1000,
+                seed=k) for k in range(8)]
+    big = Job(100, edge=5, n_carriers=2, hops=100, seed=5)  # 1500 sites, 18 MB
+    for j in jobs:
+        srv.submit(j)
+    srv.submit(big)
+    srv.drain()
+    assert all(j.result is not None and 0 < j.result["hops"] <= 1000
+               for j in jobs)
+    assert "error" in big.result
+    assert srv.evictions >= 1
+    srv.close()
+
+
+def test_row_arena():
+    a = RowArena(100)
+    x = a.alloc(40)
+    y = a.alloc(40)
+    assert (x, y) == (0, 40)
+    assert a.alloc(30) is None
+    a.release(x, 40)
+    assert a.alloc(30) == 0
+    a.release(y, 40)
+    a.release(0, 30)
+    assert a.used == 0 and a.free == [(0, 100)]
+
+
+def test_realtime_poisson_small():
+    jobs = poisson_jobs(30, rate=200.0, edges=(3, 4), nc=(1, 8),
+                        hops=(1e3, 1e4), seed=3)
+    srv = KmcServer(corundum_family, n_slots=64, nc_max=8,
+                    rows_capacity=1 << 22, slice_hops=1000, path_rows=50,
+                    msd_lags=10)
+    stats = srv.serve(jobs)
+    srv.close()
+    print(stats)
+    assert stats["jobs"] == 30
+    assert stats["p99_latency_ms"] >= stats["p50_latency_ms"] > 0
