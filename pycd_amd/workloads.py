@@ -142,13 +142,17 @@ class Workload:
 
     def __init__(self, params, size, temp=300.0, n_carriers=1, field_vpm=0.0,
                  field_dir=(0, 0, 1), field_ld=0, t_final=1e-7,
-                 time_interval=1e-12, seed=2, table_precision="f64"):
+                 time_interval=1e-12, seed=2, table_precision="f64",
+                 hop_lists=None):
+        """hop_lists(material, neighbors) may supply the hop neighbour list
+        (default: built on the GPU)."""
         self.params = params
         self.size = np.array([size] * 3 if np.isscalar(size) else size)
         self.material = Material(ReturnValues(**params))
         self.neighbors = Neighbors(self.material, self.size,
                                    np.array([1, 1, 1]))
-        self.system = System(self.material, self.neighbors, None, None,
+        hop = hop_lists(self.material, self.neighbors) if hop_lists else None
+        self.system = System(self.material, self.neighbors, hop, None,
                              params["alpha"], params["r_cut"], params["k_cut"],
                              params["precision_parameters"], [], [],
                              table_precision=table_precision)
