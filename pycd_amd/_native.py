@@ -9,7 +9,8 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "libpycd_hip.so"
+LIB_PATH = Path(os.environ.get(
+    "PYCD_LIB", Path(__file__).resolve().parent / "lib" / "libpycd_hip.so"))
 
 PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
 OK, NEED_UNIFORMS = 0, 1

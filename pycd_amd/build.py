@@ -34,32 +34,37 @@ def _hipcc():
     return str(cand) if cand.exists() else "hipcc"
 
 
-def _compile(src):
-    obj = OBJ_DIR / (src.stem + ".o")
-    cmd = [_hipcc(), *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
+def _compile(src, extra=(), obj_dir=None):
+    obj = (obj_dir or OBJ_DIR) / (src.stem + ".o")
+    cmd = [_hipcc(), *HIP_FLAGS, *extra, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr}")
     return obj
 
 
-def build(verbose=True):
-    OBJ_DIR.mkdir(exist_ok=True)
-    LIB_DIR.mkdir(exist_ok=True)
+def build(verbose=True, out=None, defines=(), obj_dir=None):
+    """Compile and link; `defines` are extra -D flags (tuning variants are
+    built to `out` with their own `obj_dir`)."""
+    obj_dir = Path(obj_dir) if obj_dir else OBJ_DIR
+    out = Path(out) if out else LIB
+    obj_dir.mkdir(parents=True, exist_ok=True)
+    out.parent.mkdir(parents=True, exist_ok=True)
     sources = sorted(SRC.glob("*.hip"))
+    extra = [f"-D{d}" for d in defines]
     workers = min(len(sources), max(1, (os.cpu_count() or 4)))
     with concurrent.futures.ThreadPoolExecutor(workers) as pool:
-        objs = list(pool.map(_compile, sources))
-    tmp = LIB.with_suffix(".so.tmp")
+        objs = list(pool.map(lambda s: _compile(s, extra, obj_dir), sources))
+    tmp = out.with_suffix(".so.tmp")
     cmd = [_hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC",
            *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {LIB} from {len(sources)} sources")
-    return LIB
+        print(f"built {out} from {len(sources)} sources")
+    return out
 
 
 if __name__ == "__main__":

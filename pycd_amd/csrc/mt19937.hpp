@@ -76,6 +76,27 @@ __device__ inline uint32_t mt_next(uint32_t* mt, int& mti, int lane) {
   return mt_temper(mt[mti++]);
 }
 
+// The next four words (two random() draws).  The common case reads them
+// straight from the block; near the block's end a loop with the one inlined
+// regeneration takes them one by one.
+__device__ inline void mt_next4(uint32_t* mt, int& mti, int lane,
+                                uint32_t w[4]) {
+  if (mti <= kMtN - 4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = mt_temper(mt[mti + k]);
+    mti += 4;
+    return;
+  }
+#pragma unroll 1
+  for (int k = 0; k < 4; ++k) {
+    if (mti >= kMtN) {
+      mt_twist(mt, lane);
+      mti = 0;
+    }
+    w[k] = mt_temper(mt[mti++]);
+  }
+}
+
 __device__ inline double mt_uniform(uint32_t a, uint32_t b) {
   return (double(a >> 5) * 67108864.0 + double(b >> 6)) *
          (1.0 / 9007199254740992.0);
