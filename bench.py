@@ -62,19 +62,24 @@ def parse():
     return ap.parse_args()
 
 
-def algorithmic_bytes_batched(trajs, fams, hops, starts, table_bytes):
-    """Bytes the batched algorithm must touch per launch.
+def algorithmic_bytes_batched(trajs, fams, hops, starts, table_bytes,
+                              refresh=1024):
+    """Bytes the batched algorithm touches per launch.
 
     Per hop of a trajectory with n_c carriers and z slots:
-      table gathers   3 (1 + z) n_c entries          (psi update + refresh)
-      per process     64 B slot record + 4 x 8 B (phi_L, E at both ends)
+      table gathers   2 (1 + z) n_c + (z - 1) n_c entries (every slot's psi
+                      update, fresh sums of the mover's new slots), plus
+                      (1 + z) n_c^2 / refresh for the periodic recompute
+      hop records     z x 72 B (the new processes) + 8 B (hop vector)
+      mover state     2 x 48 B (displacement and drift, read and written)
     plus 3 n_c doubles per path row written.
     """
     total = 0.0
     for k, t in enumerate(trajs):
         z = fams[t["family"]].z
         nc = t["n_carriers"]
-        per_hop = 3 * (1 + z) * nc * table_bytes + nc * z * (64 + 32)
+        entries = (2 * (1 + z) + (z - 1)) * nc + (1 + z) * nc * nc / refresh
+        per_hop = entries * table_bytes + z * 72 + 8 + 96
         total += per_hop * hops[k] + 24.0 * nc * starts[k]
     return total
 

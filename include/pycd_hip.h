@@ -203,15 +203,26 @@ int pycd_msd(pycd_ctx* ctx, const double* pos, int64_t n_traj,
 /* --------------------------------------------------------- batched KMC -- */
 /* Many independent trajectories (the loop over trajectories of
  * Run.do_kmc_steps, PyCD/core.py:2668, each running the loop body of
- * PyCD/core.py:2787-2861) in one persistent launch.  A family is one lattice
- * size: its table and hop slots are shared by all its trajectories.  Every
- * trajectory draws its uniforms from its own CPython MT19937 state on the
- * device (4 words per hop, as random.random() twice). */
+ * PyCD/core.py:2787-2861) in one persistent launch.  A family is one
+ * supercell: its table and hop slots are shared by all its trajectories.
+ * Every trajectory draws its uniforms from its own CPython MT19937 state on
+ * the device (4 words per hop, as random.random() twice).
+ *
+ * A family must be a periodic supercell of identical unit cells with the
+ * sites numbered cell-major, site = ((cx cells[1] + cy) cells[2] + cz)
+ * n_unit + sub (Material.generate_sites, PyCD/core.py:154-241): the library
+ * checks that every per-site value, hop slot and (with table_rows = N)
+ * table entry is the cell-0 one moved by the site's cell, and keeps only
+ * the table rows of cell 0's sites on the device (n_unit x N). */
 typedef struct {
-  int64_t n_sites;            /* N                                          */
+  int64_t n_sites;            /* N = cells[0] cells[1] cells[2] n_unit      */
   int32_t z;                  /* hop slots per hopping site (1..7)          */
   int32_t max_z;              /* row stride of the slot tables              */
-  const double* table;        /* N x N pair table                           */
+  int32_t cells[3];           /* supercell size in unit cells (1..255 each) */
+  int32_t n_unit;             /* sites per unit cell (1..255, n_unit z<=256)*/
+  int64_t table_rows;         /* N: full table, checked entry by entry;
+                                 n_unit: the rows of cell 0's sites only    */
+  const double* table;        /* table_rows x N pair table                  */
   const double* lattice_q;    /* N lattice charges                          */
   const int32_t* nbr_site;    /* N x max_z                                  */
   const double* nbr_vec;      /* N x max_z x 3                              */

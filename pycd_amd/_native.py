@@ -45,7 +45,9 @@ class KmcProblemC(C.Structure):
 
 class FamilyC(C.Structure):
     _fields_ = [("n_sites", C.c_int64), ("z", C.c_int32),
-                ("max_z", C.c_int32), ("table", C.c_void_p),
+                ("max_z", C.c_int32), ("cells", C.c_int32 * 3),
+                ("n_unit", C.c_int32), ("table_rows", C.c_int64),
+                ("table", C.c_void_p),
                 ("lattice_q", C.c_void_p), ("nbr_site", C.c_void_p),
                 ("nbr_vec", C.c_void_p), ("nbr_lambda", C.c_void_p),
                 ("nbr_vab", C.c_void_p), ("site_energy", C.c_void_p)]

@@ -4,6 +4,10 @@
 table each, shared by every trajectory of that size) on the device and runs
 any number of independent trajectories of those families in one persistent
 kernel launch, optionally replayed through a captured hipGraph.
+
+A family is a periodic supercell: the library checks that the table and
+the hop slots are invariant under cell translations and keeps only the
+table rows of the sites of cell 0 (n_unit x N) on the device.
 """
 import ctypes as C
 
@@ -17,10 +21,25 @@ MAX_CARRIERS = 64
 
 
 class Family:
-    """Host arrays of one lattice size (from a Run and its table)."""
+    """Host arrays of one lattice size (from a Run and its table).
+
+    `table` is the full N x N pair table (checked entry by entry against
+    its cell-0 rows when the family is uploaded) or only the n_unit rows of
+    the sites of cell 0.
+    """
 
     def __init__(self, run, table):
         self.run = run
+        self.cells = [int(v) for v in run.system_size]
+        n_cells = int(np.prod(self.cells))
+        table = np.asarray(table)
+        n_sites = table.shape[1]
+        if n_sites % n_cells:
+            raise ValueError("site count is not a multiple of the cell count")
+        self.n_unit = n_sites // n_cells
+        if table.shape[0] not in (n_sites, self.n_unit):
+            raise ValueError("table must be N x N or n_unit x N")
+        self.n_sites = n_sites
         self.table = _f64(table)
         self.lattice_q = _f64(run.base_charge_config()[:, 0])
         self.nbr_site = np.ascontiguousarray(run.nbr_site, dtype=np.int32)
@@ -37,11 +56,28 @@ class Family:
         self.volume = run.system.system_volume
         self.alpha = run.system.alpha
 
+    def device_bytes(self, table_precision="f64"):
+        """Device footprint of the family's compact form."""
+        rows = self.n_unit * self.n_sites
+        per = 12 if table_precision == "f32" else 8
+        return rows * per + self.n_unit * (8 + 72 * self.z) + 4 * self.n_sites
+
+    def compact(self):
+        """The same family with only the table rows of cell 0's sites."""
+        out = object.__new__(Family)
+        out.__dict__.update(self.__dict__)
+        out.table = np.ascontiguousarray(self.table[:self.n_unit])
+        return out
+
     def cstruct(self):
         f = FamilyC()
-        f.n_sites = self.table.shape[0]
+        f.n_sites = self.n_sites
         f.z = self.z
         f.max_z = self.nbr_site.shape[1]
+        for a in range(3):
+            f.cells[a] = self.cells[a]
+        f.n_unit = self.n_unit
+        f.table_rows = self.table.shape[0]
         for name in ("table", "lattice_q", "nbr_site", "nbr_vec",
                      "nbr_lambda", "nbr_vab", "site_energy"):
             setattr(f, name, getattr(self, name).ctypes.data)

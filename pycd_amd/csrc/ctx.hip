@@ -1,7 +1,7 @@
 // Context management of libpycd_hip.so.
 #include "common.hpp"
 
-extern "C" int pycd_abi_version(void) { return 1; }
+extern "C" int pycd_abi_version(void) { return 2; }
 
 extern "C" int pycd_ctx_create(int device, pycd_ctx** out) {
   if (!out) return PYCD_ERR_ARG;

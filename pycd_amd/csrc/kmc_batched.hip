@@ -2,16 +2,34 @@
 // (PyCD/core.py:2668-2861) of uneven supercell size and carrier count in
 // one persistent launch on gfx950.
 //
+// Families are cell-periodic: a supercell of nx x ny x nz unit cells with
+// n_unit sites each, numbered cell-major, site = ((cx ny + cy) nz + cz)
+// n_unit + sub (the order of Material.generate_sites, PyCD/core.py:154-241).
+// On such a lattice the pair table, the lattice potential, the site
+// energies and every hop slot are invariant under cell translations, so a
+// family keeps only the table rows of the n_unit sites of cell 0 and one
+// hop record per (sublattice, slot):
+//   A[a, b] = C[sub(a), b shifted by -cell(a)].
+// The host checks that invariance on the full table and the slot tables
+// before it accepts a family.  Only the order in which a site lists its
+// slots may differ from cell 0's (neighbour lists are sorted by site index,
+// which wraps at the supercell boundary); one word per site holds that
+// permutation (3 bits per slot), so every site keeps the slot order, and
+// with it the process order of the selection, that its own table gives.  The compact rows (0.6 MB for an 8x8x8
+// corundum supercell against 302 MB for the full table) stay in L2, so the
+// per-hop gathers are cache hits rather than HBM round trips.  Sites are
+// carried packed: cx | cy << 8 | cz << 16 | sub << 24.
+//
 // Scheduling: one wavefront (64 lanes, one workgroup) owns one trajectory at
 // a time and takes the next from a global atomic queue when it finishes, in
 // the order given by the host (longest expected work first).  The queue has
 // an exit condition every wave reaches (index >= n_traj), so the grid drains.
 //
 // Per-hop algebra is the plain path's (q.A[x,:] = phi_L[x] + psi[x]), but
-// psi is maintained incrementally: when carrier m hops s -> n, every other
-// carrier's slot values change by q_m (A[n, x] - A[s, x]) (two gathers per
-// slot) and carrier m's own z+1 slots are recomputed from scratch
-// (n_c gathers each).  That is 3 (1 + z) n_c table reads per hop.
+// psi is maintained incrementally: when carrier m hops s -> n, every slot
+// value changes by q (A[n, x] - A[s, x]) (two gathers per slot) and carrier
+// m's new slots are summed from scratch (n_c gathers each); every `refresh`
+// hops all of psi is recomputed from the table to bound the rounding.
 //
 // Random numbers: each trajectory carries its CPython MT19937 state (624
 // words + position) in LDS and draws 4 words per hop exactly like
@@ -28,28 +46,27 @@
 
 namespace pycd {
 
-struct SlotRec {      // one hop slot of one site; 64 bytes
-  int32_t nbr;        // target site, -1 if the slot does not exist
-  int32_t nbr_row;    // hopping-site row of the target (difference table)
+struct SubRec {       // hop slot j of sublattice s
+  uint32_t tgt;       // packed cell offset (each in [0, n)) | target sub << 24
+  int32_t pad0;
   double lam;         // lambda (Hartree)
   double vab;         // V_ab (Hartree)
-  double tself;       // A[s,s] - A[s,n]
+  double tself;       // A[s, s] - A[s, n]
+  double dphi;        // phi_L[n] - phi_L[s]
+  double dE;          // E[n] - E[s]
   double vec[3];      // hop vector (bohr)
-  double pad2;
 };
-static_assert(sizeof(SlotRec) == 64, "slot record is one 64-byte line");
 
 struct FamDev {
-  const double* A64;
-  const float* A32;
-  const double* D64;      // (hop row, slot) x N: A[nbr(s,j), :] - A[s, :]
-  const float* D32;
-  const int32_t* hop_row; // site -> compact hopping-site row, -1 otherwise
-  const double* phiL;
-  const double* E;
-  const SlotRec* slots;
+  const double* C64;      // n_unit x N: table rows of the sites of cell 0
+  const float* C32;
+  const SubRec* subs;     // n_unit x z
+  const uint32_t* perm;   // N: slot j of the site is slot
+                          //    (perm >> 3 j) & 7 of its sublattice
+  const double* phiL;     // n_unit
   int64_t N;
-  int32_t z, mz;
+  int32_t nx, ny, nz, nu;
+  int32_t z, pad;
   double qLAqL;
 };
 
@@ -90,7 +107,7 @@ struct BatchArgs {
   uint32_t* counter;
   // resumable state (slices): MT words, occupancy, displacement/base/drift
   uint32_t* mts;            // n_traj x 625
-  int32_t* st_occ;          // n_traj x kMaxCarriers
+  uint32_t* st_occ;         // n_traj x kMaxCarriers (packed sites)
   double* st_vec;           // n_traj x 3 x (3 kMaxCarriers)
   double* st_psi;           // n_traj x kPsiStride
   int64_t slice_hops;       // <= 0: run each trajectory to its end
@@ -99,6 +116,7 @@ struct BatchArgs {
   int32_t n_traj;
   int32_t nc_max;
   int32_t z_max;
+  int32_t sub_max;          // largest n_unit z of a family the launch uses
   double nu;
 };
 
@@ -106,10 +124,15 @@ constexpr int kMaxCarriers = 64;
 constexpr int kMaxPer = 8;    // processes per lane: n_c z <= 512
 constexpr int kPsiStride = kMaxCarriers * 8;  // n_c (1 + z), z <= 7
 constexpr int kVecStride = 3 * 3 * kMaxCarriers;
+constexpr int kMaxSubSlots = 256;  // n_unit z of a family
+constexpr int kMaxCells = 255;     // cells per axis, sites per cell
 constexpr double kEulerE = 2.718281828459045;
 
 // Values every lane holds identically, moved to scalar registers.
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint32_t uni(uint32_t v) {
+  return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+}
 __device__ inline double uni(double v) {
   const uint64_t b = __double_as_longlong(v);
   const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(b));
@@ -117,78 +140,132 @@ __device__ inline double uni(double v) {
   return __longlong_as_double((int64_t(hi) << 32) | lo);
 }
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// Cross-lane moves through DPP (rows of 16 lanes): a lane whose source is
+// outside its row (bound_ctrl) or whose row is masked off reads 0.
+template <int kCtrl, int kRowMask, bool kBound>
+__device__ inline double dpp_f64(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), kCtrl, kRowMask, 0xf,
+                                             kBound);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), kCtrl, kRowMask,
+                                             0xf, kBound);
+  return __longlong_as_double((int64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
+constexpr int kDppRowShr = 0x110;     // + n: lane i reads lane i - n of its row
+constexpr int kDppWaveShr1 = 0x138;   // lane i reads lane i - 1
+constexpr int kDppRowBcast15 = 0x142; // lane 15 of a row to the next row
+constexpr int kDppRowBcast31 = 0x143; // lane 31 to rows 2 and 3
+
+// Inclusive prefix sum over the 64 lanes: Hillis-Steele within each row of
+// 16, then the row totals carried across rows.
+__device__ inline double wave_incl_scan(double v) {
+  v += dpp_f64<kDppRowShr + 1, 0xf, true>(v);
+  v += dpp_f64<kDppRowShr + 2, 0xf, true>(v);
+  v += dpp_f64<kDppRowShr + 4, 0xf, true>(v);
+  v += dpp_f64<kDppRowShr + 8, 0xf, true>(v);
+  v += dpp_f64<kDppRowBcast15, 0xa, false>(v);
+  v += dpp_f64<kDppRowBcast31, 0xc, false>(v);
   return v;
 }
 
-__device__ inline double wave_excl_scan(double v, int lane) {
-  double incl = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double o = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += o;
-  }
-  const double excl = __shfl_up(incl, 1, 64);
-  return lane == 0 ? 0.0 : excl;
+__device__ inline double read_lane(double v, int lane) {
+  const int64_t b = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane(int(b), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(int(b >> 32), lane);
+  return __longlong_as_double((int64_t(hi) << 32) | lo);
+}
+
+// Sum over the wavefront, returned in a scalar register.
+__device__ inline double wave_sum(double v) {
+  return read_lane(wave_incl_scan(v), 63);
+}
+
+__device__ inline double wave_excl_scan(double v) {
+  return dpp_f64<kDppWaveShr1, 0xf, true>(wave_incl_scan(v));
+}
+
+// ---- packed sites of a cell-periodic family
+struct Geo {
+  int nx, ny, nz, nu;
+};
+
+__host__ __device__ inline uint32_t site_pack(uint32_t i, const Geo& g) {
+  const uint32_t sub = i % uint32_t(g.nu);
+  uint32_t c = i / uint32_t(g.nu);
+  const uint32_t cz = c % uint32_t(g.nz);
+  c /= uint32_t(g.nz);
+  const uint32_t cy = c % uint32_t(g.ny);
+  const uint32_t cx = c / uint32_t(g.ny);
+  return cx | (cy << 8) | (cz << 16) | (sub << 24);
+}
+
+__host__ __device__ inline uint32_t site_index(uint32_t p, const Geo& g) {
+  return ((((p & 0xffu) * uint32_t(g.ny) + ((p >> 8) & 0xffu)) *
+               uint32_t(g.nz) +
+           ((p >> 16) & 0xffu)) *
+          uint32_t(g.nu)) +
+         (p >> 24);
+}
+
+// Site s moved by the packed cell offset d (components in [0, n)), landing
+// on the sublattice that d names.
+__host__ __device__ inline uint32_t site_add(uint32_t s, uint32_t d,
+                                             const Geo& g) {
+  int x = int(s & 0xffu) + int(d & 0xffu);
+  x -= x >= g.nx ? g.nx : 0;
+  int y = int((s >> 8) & 0xffu) + int((d >> 8) & 0xffu);
+  y -= y >= g.ny ? g.ny : 0;
+  int w = int((s >> 16) & 0xffu) + int((d >> 16) & 0xffu);
+  w -= w >= g.nz ? g.nz : 0;
+  return uint32_t(x) | (uint32_t(y) << 8) | (uint32_t(w) << 16) |
+         (d & 0xff000000u);
+}
+
+// Column of site b in the compact row of a's sublattice: b moved by
+// -cell(a), as a site index.
+__host__ __device__ inline uint32_t rel_col(uint32_t a, uint32_t b,
+                                            const Geo& g) {
+  int dx = int(b & 0xffu) - int(a & 0xffu);
+  dx += dx < 0 ? g.nx : 0;
+  int dy = int((b >> 8) & 0xffu) - int((a >> 8) & 0xffu);
+  dy += dy < 0 ? g.ny : 0;
+  int dz = int((b >> 16) & 0xffu) - int((a >> 16) & 0xffu);
+  dz += dz < 0 ? g.nz : 0;
+  return uint32_t(((dx * g.ny + dy) * g.nz + dz) * g.nu) + (b >> 24);
 }
 
 template <typename TT>
-__device__ inline double tab(const FamDev& f, int64_t idx);
+__device__ inline const TT* crow(const FamDev& f, uint32_t sub);
 template <>
-__device__ inline double tab<double>(const FamDev& f, int64_t idx) {
-  return f.A64[idx];
+__device__ inline const double* crow<double>(const FamDev& f, uint32_t sub) {
+  return f.C64 + int64_t(sub) * f.N;
 }
 template <>
-__device__ inline double tab<float>(const FamDev& f, int64_t idx) {
-  return double(f.A32[idx]);
-}
-
-// Row pointers (uniform across the wavefront) for the gathers.
-template <typename TT>
-__device__ inline const TT* trow(const FamDev& f, int64_t row);
-template <>
-__device__ inline const double* trow<double>(const FamDev& f, int64_t row) {
-  return f.A64 + row * f.N;
-}
-template <>
-__device__ inline const float* trow<float>(const FamDev& f, int64_t row) {
-  return f.A32 + row * f.N;
-}
-template <typename TT>
-__device__ inline const TT* drow_ptr(const FamDev& f, int64_t row);
-template <>
-__device__ inline const double* drow_ptr<double>(const FamDev& f, int64_t row) {
-  return f.D64 + row * f.N;
-}
-template <>
-__device__ inline const float* drow_ptr<float>(const FamDev& f, int64_t row) {
-  return f.D32 + row * f.N;
+__device__ inline const float* crow<float>(const FamDev& f, uint32_t sub) {
+  return f.C32 + int64_t(sub) * f.N;
 }
 
+// A[a, b] for packed sites a, b.
 template <typename TT>
-__device__ inline double dtab(const FamDev& f, int64_t idx);
-template <>
-__device__ inline double dtab<double>(const FamDev& f, int64_t idx) {
-  return f.D64[idx];
-}
-template <>
-__device__ inline double dtab<float>(const FamDev& f, int64_t idx) {
-  return double(f.D32[idx]);
+__device__ inline double pair_entry(const FamDev& f, const Geo& g, uint32_t a,
+                                    uint32_t b) {
+  return double(crow<TT>(f, a >> 24)[rel_col(a, b, g)]);
 }
 
 // psi[r] = sum_c q A[sites[r], occ[c]] for every slot r (lanes over slots).
 template <typename TT>
-__device__ inline void psi_from_table(const FamDev& f, double q, int R, int nc,
-                                      const int32_t* sites,
-                                      const int32_t* occ, double* psi,
+__device__ inline void psi_from_table(const FamDev& f, const Geo& g, double q,
+                                      int R, int nc, const uint32_t* sites,
+                                      const uint32_t* occ, double* psi,
                                       int lane) {
   for (int r = lane; r < R; r += 64) {
-    const int64_t row = int64_t(sites[r]) * f.N;
+    const uint32_t a = sites[r];
+    const TT* row = crow<TT>(f, a >> 24);
     double acc = 0.0;
 #pragma unroll 4
-    for (int c2 = 0; c2 < nc; ++c2) acc = fma(q, tab<TT>(f, row + occ[c2]), acc);
+    for (int c2 = 0; c2 < nc; ++c2)
+      acc = fma(q, double(row[rel_col(a, occ[c2], g)]), acc);
     psi[r] = acc;
   }
 }
@@ -199,9 +276,12 @@ __device__ inline void psi_from_table(const FamDev& f, double q, int R, int nc,
 //   psi    rmax doubles              potential of the carriers at each slot
 //   pcd    pmax x kPcD doubles       static part of each process's dG0
 //   pcf    pmax x 2 floats           (FP32 rates) lambda, V_ab + field shift
-//   sites  rmax ints                 slot sites (carrier site, its targets)
-//   nn     pmax x z_max ints         hop targets of each process's target
-//   occ, ohr  nc_max ints each       carrier sites and their hop rows
+//   sites  rmax words                packed slot sites (carrier, targets)
+//   occ    nc_max words              packed carrier sites
+//   cperm  nc_max words              slot permutation of each carrier's site
+//   tg     sub_max words             the family's hop targets (sub, slot)
+//   pperm  pmax words                slot permutation of each process's
+//                                    target site
 template <bool kF32Rate>
 struct PcLayout {
   static constexpr int kPcD = kF32Rate ? 1 : 6;
@@ -209,12 +289,37 @@ struct PcLayout {
 };
 
 __host__ __device__ inline size_t batch_smem_bytes(int nc_max, int z_max,
-                                                  bool f32rate) {
+                                                  bool f32rate, int sub_max) {
   const size_t rmax = size_t(nc_max) * (1 + z_max);
   const size_t pmax = size_t(nc_max) * z_max;
   const size_t pcd = f32rate ? 1 : 6, pcf = f32rate ? 2 : 0;
   return 2560 + 8 * rmax + 8 * pmax * pcd + 4 * pmax * pcf + 4 * rmax +
-         4 * pmax * z_max + 8 * size_t(nc_max);
+         8 * size_t(nc_max) + 4 * size_t(sub_max) + 4 * pmax;
+}
+
+// Static part of process p's dG0 and rate inputs, from its hop record.
+template <bool kF32Rate>
+__device__ inline void fill_process(double* pcd, float* pcf, int p,
+                                    const SubRec& rec, double q,
+                                    const double* field, bool field_on) {
+  const double tq = q * rec.tself;
+  const double shift =
+      field_on ? 0.5 * (field[0] * rec.vec[0] + field[1] * rec.vec[1] +
+                        field[2] * rec.vec[2])
+               : 0.0;
+  if (kF32Rate) {
+    pcd[p] = 2 * q * (rec.dphi + tq) + rec.dE;
+    pcf[2 * p] = float(rec.lam);
+    pcf[2 * p + 1] = float(rec.vab) + float(shift);
+  } else {
+    double* e = pcd + 6 * p;
+    e[0] = rec.dphi;
+    e[1] = tq;
+    e[2] = rec.dE;
+    e[3] = rec.lam;
+    e[4] = rec.vab;
+    e[5] = shift;
+  }
 }
 
 // kPer: processes per lane (n_c z <= 64 kPer); kZp: slots per carrier
@@ -230,22 +335,22 @@ kmc_batched_kernel(BatchArgs a) {
   using L = PcLayout<kF32Rate>;
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x;
-  const int zmax = a.z_max;
-  const int rmax = a.nc_max * (1 + zmax);
-  const int pmax = a.nc_max * zmax;
+  const int rmax = a.nc_max * (1 + a.z_max);
+  const int pmax = a.nc_max * a.z_max;
   uint32_t* mt = reinterpret_cast<uint32_t*>(smem_raw);
   double* psi = reinterpret_cast<double*>(smem_raw + 2560);
   double* pcd = psi + rmax;
   float* pcf = reinterpret_cast<float*>(pcd + size_t(pmax) * L::kPcD);
-  int32_t* sites = reinterpret_cast<int32_t*>(pcf + size_t(pmax) * L::kPcF);
-  int32_t* nn = sites + rmax;
-  int32_t* occ = nn + size_t(pmax) * zmax;
-  int32_t* ohr = occ + a.nc_max;
+  uint32_t* sites = reinterpret_cast<uint32_t*>(pcf + size_t(pmax) * L::kPcF);
+  uint32_t* occ = sites + rmax;
+  uint32_t* cperm = occ + a.nc_max;
+  uint32_t* tg = cperm + a.nc_max;
+  uint32_t* pperm = tg + a.sub_max;
 
   for (;;) {
     uint32_t ti = 0;
     if (lane == 0) ti = atomicAdd(a.counter, 1u);
-    ti = uint32_t(uni(int(__shfl(ti, 0, 64))));
+    ti = uni(uint32_t(__shfl(int(ti), 0, 64)));
     if (ti >= uint32_t(a.n_traj)) break;
     const int tr = uni(a.order[ti]);
     const TrajOut o0 = a.out[tr];
@@ -253,10 +358,10 @@ kmc_batched_kernel(BatchArgs a) {
     const bool resume = uni(o0.started) != 0;
     const TrajDev td = a.trajs[tr];
     const FamDev f = a.fams[uni(td.fam)];
+    const Geo g{f.nx, f.ny, f.nz, f.nu};
     const int nc = td.nc, z = f.z, zp = z + 1;
     const int R = nc * zp, nproc = nc * z;
     const int per = (nproc + 63) / 64;
-    const int64_t N = f.N;
     const double q = td.q;
     double* rows = a.rows + td.row_off;
     const int w = 3 * nc;
@@ -266,7 +371,7 @@ kmc_batched_kernel(BatchArgs a) {
     // per-trajectory state in global memory: each entry k of disp, base and
     // drift is read and written only by lane k % 64
     uint32_t* mts = a.mts + int64_t(tr) * 625;
-    int32_t* socc = a.st_occ + int64_t(tr) * kMaxCarriers;
+    uint32_t* socc = a.st_occ + int64_t(tr) * kMaxCarriers;
     double* svec = a.st_vec + int64_t(tr) * kVecStride;
     double* sdisp = svec;
     double* sbase = svec + 3 * kMaxCarriers;
@@ -278,9 +383,12 @@ kmc_batched_kernel(BatchArgs a) {
     for (int k = lane; k < 624; k += 64) mt[k] = m0[k];
     int mti = int(m0[624]);
     for (int c = lane; c < nc; c += 64) {
-      occ[c] = resume ? socc[c] : int32_t(a.occ0[td.occ_off + c]);
-      ohr[c] = f.hop_row[occ[c]];
+      const uint32_t s =
+          resume ? socc[c] : site_pack(uint32_t(a.occ0[td.occ_off + c]), g);
+      occ[c] = s;
+      cperm[c] = f.perm[site_index(s, g)];
     }
+    for (int k = lane; k < f.nu * z; k += 64) tg[k] = f.subs[k].tgt;
     if (!resume)
       for (int k = lane; k < w; k += 64) {
         sdisp[k] = 0.0;
@@ -291,43 +399,19 @@ kmc_batched_kernel(BatchArgs a) {
     __syncthreads();
     for (int r = lane; r < R; r += 64) {
       const int c = r / zp, k = r - c * zp;
-      const int s = occ[c];
-      sites[r] = k == 0 ? s : f.slots[int64_t(s) * f.mz + k - 1].nbr;
+      const uint32_t s = occ[c];
+      const uint32_t jj = (cperm[c] >> (3 * (k - 1))) & 7u;
+      sites[r] = k == 0 ? s : site_add(s, tg[(s >> 24) * z + jj], g);
     }
-    __syncthreads();
-    // process cache and second neighbours of every process
+    // process cache and the slot permutations of the targets
     for (int p = lane; p < nproc; p += 64) {
       const int c = p / z, j = p - c * z;
-      const int s = sites[c * zp];
-      const SlotRec& rec = f.slots[int64_t(s) * f.mz + j];
-      const int n = rec.nbr;
-      const double dphi = f.phiL[n] - f.phiL[s];
-      const double tq = q * rec.tself;
-      const double dE = f.E[n] - f.E[s];
-      const double shift =
-          field_on ? 0.5 * (td.field[0] * rec.vec[0] + td.field[1] * rec.vec[1] +
-                            td.field[2] * rec.vec[2])
-                   : 0.0;
-      if (kF32Rate) {
-        pcd[p] = 2 * q * (dphi + tq) + dE;
-        pcf[2 * p] = float(rec.lam);
-        pcf[2 * p + 1] = float(rec.vab) + float(shift);
-      } else {
-        double* e = pcd + 6 * p;
-        e[0] = dphi;
-        e[1] = tq;
-        e[2] = dE;
-        e[3] = rec.lam;
-        e[4] = rec.vab;
-        e[5] = shift;
-      }
+      const uint32_t s = occ[c];
+      const uint32_t k = (s >> 24) * z + ((cperm[c] >> (3 * j)) & 7u);
+      fill_process<kF32Rate>(pcd, pcf, p, f.subs[k], q, td.field, field_on);
+      pperm[p] = f.perm[site_index(site_add(s, tg[k], g), g)];
     }
-    for (int pk = lane; pk < nproc * z; pk += 64) {
-      const int p = pk / z, k = pk - p * z;
-      const int c = p / z, j = p - c * z;
-      const int t = sites[c * zp + 1 + j];
-      nn[p * zmax + k] = f.slots[int64_t(t) * f.mz + k].nbr;
-    }
+    __syncthreads();
     double t_clk, energy;
     int64_t start, hops;
     int no_select;
@@ -339,15 +423,15 @@ kmc_batched_kernel(BatchArgs a) {
       hops = o0.hops;
       no_select = o0.no_select;
     } else {
-      psi_from_table<TT>(f, q, R, nc, sites, occ, psi, lane);
+      psi_from_table<TT>(f, g, q, R, nc, sites, occ, psi, lane);
       // initial energy: ewald_neut + q_L A q_L + sum_c q (2 phi_L + psi_self)
       double e_loc = 0.0;
       for (int c = lane; c < nc; c += 64) {
-        const int64_t row = int64_t(occ[c]) * N;
+        const uint32_t s = occ[c];
         double inner = 0.0;
         for (int c2 = 0; c2 < nc; ++c2)
-          inner = fma(q, tab<TT>(f, row + occ[c2]), inner);
-        e_loc += q * (2.0 * f.phiL[occ[c]] + inner);
+          inner = fma(q, pair_entry<TT>(f, g, s, occ[c2]), inner);
+        e_loc += q * (2.0 * f.phiL[s >> 24] + inner);
       }
       energy = td.ewald_neut + (f.qLAqL + wave_sum(e_loc));
       t_clk = 0.0;
@@ -370,9 +454,16 @@ kmc_batched_kernel(BatchArgs a) {
       ++slice_done;
       if (a.refresh > 0 && hops > 0 && hops % a.refresh == 0) {
         // bound the rounding carried by the incremental updates
-        psi_from_table<TT>(f, q, R, nc, sites, occ, psi, lane);
+        psi_from_table<TT>(f, g, q, R, nc, sites, occ, psi, lane);
         __syncthreads();
       }
+      // ---- two uniforms, as random.random() draws them; the clock's
+      // logarithm does not depend on the rates and overlaps them
+      uint32_t wd[4];
+      mt_next4(mt, mti, lane, wd);
+      const double u1 = mt_uniform(wd[0], wd[1]);
+      const double u2 = mt_uniform(wd[2], wd[3]);
+      const double lu2 = u2 > 0.0 ? log(u2) : 0.0;
       // ---- rates from shared memory only
       double kv[kPer], gv[kPer];
       double ksum = 0.0;
@@ -405,24 +496,23 @@ kmc_batched_kernel(BatchArgs a) {
           ksum += k;
         }
       }
-      const double k_tot = uni(wave_sum(ksum));
-      // ---- two uniforms, as random.random() draws them
-      uint32_t wd[4];
-      mt_next4(mt, mti, lane, wd);
-      const double u1 = mt_uniform(wd[0], wd[1]);
-      const double u2 = mt_uniform(wd[2], wd[3]);
+      const double k_tot = wave_sum(ksum);
       // ---- FP64 prefix of k / k_tot and selection
+      double xq[kPer];
       double xs = 0.0;
 #pragma unroll
-      for (int i = 0; i < kPer; ++i) xs += kv[i] / k_tot;
-      double cum = wave_excl_scan(xs, lane);
+      for (int i = 0; i < kPer; ++i) {
+        xq[i] = kv[i] / k_tot;
+        xs += xq[i];
+      }
+      double cum = wave_excl_scan(xs);
       int found = -1;
       double g_found = 0.0, k_found = 0.0;
 #pragma unroll
       for (int i = 0; i < kPer; ++i) {
         const int p = lane * per + i;
         if (i < per && p < nproc) {
-          cum += kv[i] / k_tot;
+          cum += xq[i];
           if (found < 0 && cum > u1) {
             found = p;
             g_found = gv[i];
@@ -449,66 +539,71 @@ kmc_batched_kernel(BatchArgs a) {
         }
         ++no_select;
       }
-      const int sel = uni(__shfl(found, src, 64));
-      const double g = uni(__shfl(g_found, src, 64));
-      const double ksel = uni(__shfl(k_found, src, 64));
+      src = uni(src);
+      const int sel = __builtin_amdgcn_readlane(found, src);
+      const double gsel = read_lane(g_found, src);
+      const double ksel = read_lane(k_found, src);
       // ---- clock
       int64_t end;
       if (u2 > 0.0) {
-        t_clk -= log(u2) / k_tot;
+        t_clk -= lu2 / k_tot;
         const double qq = t_clk / td.dt;
         end = qq < 9.0e18 ? int64_t(qq) : td.P;
       } else {
         t_clk = INFINITY;
         end = td.P;
       }
-      energy += g;
+      energy += gsel;
       // ---- one round of independent loads for the whole update
       const int m = sel / z;
       const int j = sel - m * z;
-      const int s_old = uni(sites[m * zp]);
-      const int s_new = uni(sites[m * zp + 1 + j]);
-      const TT* drp = drow_ptr<TT>(f, int64_t(uni(ohr[m])) * z + j);
-      int nsel[kZp - 1];  // targets of the new site, from shared memory
+      const uint32_t s_old = uni(sites[m * zp]);
+      const uint32_t s_new = uni(sites[m * zp + 1 + j]);
+      const uint32_t sub_new = s_new >> 24;
+      const uint32_t perm_new = uni(pperm[sel]);
+      const uint32_t rec_old = (s_old >> 24) * z + ((uni(cperm[m]) >> (3 * j)) & 7u);
+      // hop records of the new site and the slot permutations of its
+      // targets (lanes < z), issued first: the next process cache
+      SubRec rec_new;
+      uint32_t tperm = 0;
+      if (lane < z) {
+        const uint32_t k = sub_new * z + ((perm_new >> (3 * lane)) & 7u);
+        rec_new = f.subs[k];
+        tperm = f.perm[site_index(site_add(s_new, tg[k], g), g)];
+      }
+      uint32_t nsel[kZp - 1];  // targets of the new site, in its slot order
 #pragma unroll
       for (int k = 0; k < kZp - 1; ++k)
-        nsel[k] = k < z ? uni(nn[sel * zmax + k]) : -1;
+        nsel[k] = k < z ? uni(site_add(
+                              s_new, tg[sub_new * z + ((perm_new >> (3 * k)) & 7u)], g))
+                        : 0u;
+      const TT* rn = crow<TT>(f, sub_new);
+      const TT* ro = crow<TT>(f, s_old >> 24);
       constexpr int kRPer = kMaxCarriers * kZp / 64;
       double dv[kRPer];
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
         const int r = lane + 64 * i;
-        dv[i] = r < R ? double(drp[uint32_t(sites[r])]) : 0.0;
+        dv[i] = 0.0;
+        if (r < R) {
+          const uint32_t x = sites[r];
+          dv[i] = double(rn[rel_col(s_new, x, g)]) -
+                  double(ro[rel_col(s_old, x, g)]);
+        }
       }
       // fresh sums of the new slots (not the one that returns to s_old):
       // one carrier per lane (n_c <= 64)
       double part[kZp - 1];
       {
-        const int o = lane < nc ? (lane == m ? s_new : occ[lane]) : -1;
+        const bool has = lane < nc;
+        const uint32_t o = has ? (lane == m ? s_new : occ[lane]) : 0u;
 #pragma unroll
         for (int k = 0; k < kZp - 1; ++k) {
           part[k] = 0.0;
-          if (k < z && o >= 0 && nsel[k] != s_old)
-            part[k] = q * double(trow<TT>(f, nsel[k])[uint32_t(o)]);
+          if (k < z && has && nsel[k] != s_old)
+            part[k] = q * pair_entry<TT>(f, g, nsel[k], o);
         }
       }
-      // slot records of the new site (lanes < z): next process cache
-      SlotRec rec_new;
-      double phin = 0.0, en = 0.0, phis = 0.0, es = 0.0;
-      if (lane < z) {
-        rec_new = f.slots[int64_t(s_new) * f.mz + lane];
-        phin = f.phiL[rec_new.nbr];
-        en = f.E[rec_new.nbr];
-        phis = f.phiL[s_new];
-        es = f.E[s_new];
-      }
-      // second neighbours of the new processes (lanes < z z)
-      int nn_new = -1;
-      if (lane < z * z) {
-        const int jj = lane / z, kk = lane - jj * z;
-        nn_new = f.slots[int64_t(nsel[jj < kZp - 1 ? jj : 0]) * f.mz + kk].nbr;
-      }
-      const int nrow = uni(f.slots[int64_t(s_old) * f.mz + j].nbr_row);
       // displacement by the owner lanes of carrier m's three components
       double vec_own = 0.0;
       int own_k = -1;  // the component of carrier m this lane owns, if any
@@ -516,9 +611,14 @@ kmc_batched_kernel(BatchArgs a) {
       for (int cc = 0; cc < 3; ++cc)
         if (lane == (3 * m + cc) % 64) own_k = 3 * m + cc;
       const bool owner = own_k >= 0;
-      if (owner)
-        vec_own = f.slots[int64_t(s_old) * f.mz + j].vec[own_k - 3 * m];
-      // ---- apply
+      if (owner) vec_own = f.subs[rec_old].vec[own_k - 3 * m];
+      // ---- apply: the new processes first (their records arrived first;
+      // process m's old entries are dead once it is selected)
+      if (lane < z) {
+        fill_process<kF32Rate>(pcd, pcf, m * z + lane, rec_new, q, td.field,
+                               field_on);
+        pperm[m * z + lane] = tperm;
+      }
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
         const int r = lane + 64 * i;
@@ -538,7 +638,7 @@ kmc_batched_kernel(BatchArgs a) {
       __syncthreads();
       if (lane == 0) {
         occ[m] = s_new;
-        ohr[m] = nrow;
+        cperm[m] = perm_new;
         sites[m * zp] = s_new;
         psi[m * zp] = v_new;
 #pragma unroll
@@ -547,34 +647,6 @@ kmc_batched_kernel(BatchArgs a) {
             sites[m * zp + 1 + k] = nsel[k];
             psi[m * zp + 1 + k] = fresh_v[k];
           }
-      }
-      if (lane < z) {
-        const int pn = m * z + lane;
-        const double dphi = phin - phis;
-        const double tq = q * rec_new.tself;
-        const double dE = en - es;
-        const double shift =
-            field_on ? 0.5 * (td.field[0] * rec_new.vec[0] +
-                              td.field[1] * rec_new.vec[1] +
-                              td.field[2] * rec_new.vec[2])
-                     : 0.0;
-        if (kF32Rate) {
-          pcd[pn] = 2 * q * (dphi + tq) + dE;
-          pcf[2 * pn] = float(rec_new.lam);
-          pcf[2 * pn + 1] = float(rec_new.vab) + float(shift);
-        } else {
-          double* e = pcd + 6 * pn;
-          e[0] = dphi;
-          e[1] = tq;
-          e[2] = dE;
-          e[3] = rec_new.lam;
-          e[4] = rec_new.vab;
-          e[5] = shift;
-        }
-      }
-      if (lane < z * z) {
-        const int jj = lane / z, kk = lane - jj * z;
-        nn[(m * z + jj) * zmax + kk] = nn_new;
       }
       // ---- path rows [start, end), written by the owner lane of each entry
       ++hops;
@@ -712,38 +784,6 @@ __global__ void fill_f64(double* p, int64_t n, double v) {
 }
 
 // Family preparation kernels.
-__global__ void __launch_bounds__(256)
-row_dot_f64(const double* __restrict__ A, const double* __restrict__ q,
-            int64_t N, double* __restrict__ out) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int64_t i = int64_t(blockIdx.x) * 4 + wave; i < N;
-       i += int64_t(gridDim.x) * 4) {
-    const double* row = A + i * N;
-    double acc = 0.0;
-    for (int64_t j = lane; j < N; j += 64) acc = fma(row[j], q[j], acc);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) out[i] = acc;
-  }
-}
-
-__global__ void diff_table(const double* __restrict__ A,
-                           const SlotRec* __restrict__ slots,
-                           const int32_t* __restrict__ hop_sites, int64_t n_hop,
-                           int z, int mz, int64_t N, double* __restrict__ D) {
-  const int64_t total = n_hop * z * N;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t x = i % N;
-    const int64_t hj = i / N;
-    const int64_t h = hj / z;
-    const int j = int(hj - h * z);
-    const int64_t s = hop_sites[h];
-    const int n = slots[s * mz + j].nbr;
-    D[i] = A[int64_t(n) * N + x] - A[s * N + x];
-  }
-}
-
 __global__ void to_f32(const double* __restrict__ a, float* __restrict__ b,
                        int64_t n) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -751,28 +791,28 @@ __global__ void to_f32(const double* __restrict__ a, float* __restrict__ b,
     b[i] = float(a[i]);
 }
 
-__global__ void slot_self_terms(const double* __restrict__ A, int64_t N,
-                                SlotRec* __restrict__ slots, int mz) {
-  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= N * mz) return;
-  const int64_t s = idx / mz;
-  const int n = slots[idx].nbr;
-  slots[idx].tself = n >= 0 ? A[s * N + s] - A[s * N + n] : 0.0;
-}
-
-__global__ void dot_f64(const double* __restrict__ a,
-                        const double* __restrict__ b, int64_t n,
-                        double* __restrict__ out) {
-  __shared__ double part[256];
-  double acc = 0.0;
-  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) acc = fma(a[k], b[k], acc);
-  part[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
-    __syncthreads();
+// Largest |A[i, j] - C[sub(i), rel(i, j)]| over the full table A, whose
+// first n_unit rows are C, as the bit pattern of a non-negative double
+// (ordered like the values).  One workgroup of 256 per 4 rows.
+__global__ void __launch_bounds__(256)
+compact_check(const double* __restrict__ A, int64_t N, Geo g,
+              unsigned long long* __restrict__ worst) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double m = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * 4 + wave; i < N;
+       i += int64_t(gridDim.x) * 4) {
+    const uint32_t a = site_pack(uint32_t(i), g);
+    const double* row = A + i * N;
+    const double* crow0 = A + int64_t(a >> 24) * N;
+    for (int64_t j = lane; j < N; j += 64) {
+      const uint32_t b = site_pack(uint32_t(j), g);
+      m = fmax(m, fabs(row[j] - crow0[rel_col(a, b, g)]));
+    }
   }
-  if (threadIdx.x == 0) out[0] = part[0];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if (lane == 0)
+    atomicMax(worst, static_cast<unsigned long long>(__double_as_longlong(m)));
 }
 
 }  // namespace pycd
@@ -781,15 +821,16 @@ __global__ void dot_f64(const double* __restrict__ a,
 using namespace pycd;
 
 struct BatchFamily {
-  DevBuf<double> A64, phiL, E, qL, tmp, D64;
-  DevBuf<float> A32, D32;
-  DevBuf<int32_t> hop_row, hop_sites;
-  int64_t n_hop = 0;
-  DevBuf<SlotRec> slots;
+  DevBuf<double> C64, phiL;
+  DevBuf<float> C32;
+  DevBuf<SubRec> subs;
+  DevBuf<uint32_t> perm;
   int64_t N = 0;
   int z = 0, mz = 0;
+  Geo g{1, 1, 1, 1};
   double qLAqL = 0.0;
-  std::vector<int32_t> nbr_host;  // for host-side validation
+  std::vector<char> hop_sub;  // per sublattice: its sites hop
+  size_t device_bytes = 0;
 };
 
 struct pycd_batch {
@@ -808,7 +849,7 @@ struct pycd_batch {
   DevBuf<double> d_rows, d_drift, d_msd;
   DevBuf<TrajOut> d_out;
   DevBuf<uint32_t> d_mts;
-  DevBuf<int32_t> d_socc;
+  DevBuf<uint32_t> d_socc;
   DevBuf<double> d_svec, d_spsi;
   // slot pool (serving)
   bool pool = false;
@@ -820,6 +861,7 @@ struct pycd_batch {
   std::vector<int64_t> msd_off;
   int64_t n_traj = 0, rows_total = 0, msd_total = 0;
   int nc_max = 0, z_max = 0;
+  int sub_max = kMaxSubSlots;
   int nlb_max = 0;
   double msd_scale = 1.0;
   size_t smem = 0;
@@ -833,3 +875,4 @@ struct pycd_batch {
 };
 
 #include "batch_api.inc"
+

@@ -151,14 +151,14 @@ class KmcServer:
             self.lru.move_to_end(edge)
             return self.families[edge][0]
         run, table = self.build_family(edge)
-        need = table.nbytes
+        fam = Family(run, table)
+        need = fam.device_bytes("f64")
         busy = self._busy_edges()
         while sum(self.family_bytes.values()) + need > self.table_budget:
             victims = [e for e in self.lru if e not in busy]
             if not victims:
                 return None
             self._drop(victims[0])
-        fam = Family(run, table)
         self._ensure_batch(fam.nu if self._nu is None else self._nu)
         idx = C.c_int32()
         _check(self.ctx, lib().pycd_batch_add_family(self.h, fam.cstruct(),
@@ -201,8 +201,9 @@ class KmcServer:
         return None
 
     def table_bytes(self, edge):
+        """Device bytes of the family's compact form (cell-0 table rows)."""
         run, table = self.build_family(edge)
-        return table.nbytes
+        return Family(run, table).device_bytes("f64")
 
     def _try_admit(self, job):
         if not self.free_slots:

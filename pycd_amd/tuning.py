@@ -30,8 +30,13 @@ def lpt_order(trajs, fams):
 
 def autotune_waves(batch, trajs, occs, words, order, candidates=(2, 4, 6, 8,
                                                                   12, 16),
-                   sample=1024, hops=2000, log=None):
-    """Time `hops`-hop runs of the first `sample` queued trajectories."""
+                   sample=4096, hops=2000, log=None):
+    """Time `hops`-hop runs of the first `sample` queued trajectories.
+
+    The sample must hold several trajectories per wavefront at the largest
+    candidate, or every candidate past n_traj / CUs times one trajectory
+    per wave and the comparison says nothing; the default takes the whole
+    batch of the benchmark workload, each cut to `hops` hops."""
     sub = [int(i) for i in order[:sample]]
     sub_trajs = []
     for i in sub:

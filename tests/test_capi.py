@@ -26,7 +26,7 @@ def test_library_exports_symbols():
     lib = ctypes.CDLL(str(_native.LIB_PATH))
     for name in declared_symbols():
         assert hasattr(lib, name), name
-    assert lib.pycd_abi_version() == 1
+    assert lib.pycd_abi_version() == 2
 
 
 def test_errors_without_context():

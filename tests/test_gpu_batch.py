@@ -75,8 +75,12 @@ def test_fp64_batch_matches_oracle_per_trajectory():
         assert abs(res["energy"][k] - ora["energy_now"]) < 1e-9 * scale, k
         assert res["start"][k] == ora["start"]
         nc = spec[1]
+        # the batched family uses cell 0's hop vectors for every site (equal
+        # to each site's own to rounding), so a component that cancels to
+        # ~1e-17 is compared against the drift's magnitude
+        dscale = np.max(np.abs(ora["drift"]))
         assert np.allclose(res["drift"][k, :nc], ora["drift"], rtol=1e-9,
-                           atol=1e-300)
+                           atol=1e-12 * dscale)
     bt.close()
 
 
