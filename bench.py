@@ -121,12 +121,17 @@ def run_batched(args, log):
     ms = float(np.mean(times))
     value = total_hops / (ms / 1e3)
     # kernel-only timing (no graph) for the roofline of the dominant kernel
-    bt_k = bt.run(use_graph=False)
+    bt.run(use_graph=False)
+    bt_k = bt.kernel_ms()
+    busy = res["busy_s"]
+    log(f"  kernel {bt_k:.1f} ms; longest trajectory {busy.max() * 1e3:.1f} ms; "
+        f"busy sum / resident waves {busy.sum() / (wpc * 256) * 1e3:.1f} ms")
     algo = algorithmic_bytes_batched(trajs, fams, hops, res["start"], 8)
     achieved = algo / (bt_k / 1e3) / 1e9
     info = {
         "ms_per_step": ms, "total_hops": total_hops, "hops_per_s": value,
         "waves_per_cu": wpc, "no_select": int(res["no_select"].sum()),
+        "longest_trajectory_ms": float(busy.max() * 1e3),
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -250,6 +250,7 @@ typedef struct {
   int64_t start;              /* path rows written: [0, start)              */
   int32_t no_select;          /* hops where rounding left no cum > u1       */
   int32_t done;
+  double busy_s;              /* device time its wavefront spent on it      */
 } pycd_kmc_traj_result;
 
 typedef struct pycd_batch pycd_batch;
@@ -302,6 +303,10 @@ int pycd_batch_step(pycd_batch* b, int64_t slice_hops, double* elapsed_ms);
  * scaled by `scale` first), concatenated. */
 int pycd_batch_slot_msd(pycd_batch* b, const int64_t* slots, int64_t n,
                         const int64_t* lags, double scale, double* out);
+
+/* Device time of the KMC kernel alone (no MSD, no resets) in the last
+ * pycd_batch_run without a graph, or the last pycd_batch_step. */
+int pycd_batch_kernel_ms(const pycd_batch* b, double* ms);
 
 /* Per-trajectory results and drift sums (n_traj x 64 x 3, carrier-major). */
 int pycd_batch_results(pycd_batch* b, pycd_kmc_traj_result* results,

@@ -64,7 +64,8 @@ class TrajC(C.Structure):
 class TrajResultC(C.Structure):
     _fields_ = [("t", C.c_double), ("energy", C.c_double),
                 ("hops", C.c_int64), ("start", C.c_int64),
-                ("no_select", C.c_int32), ("done", C.c_int32)]
+                ("no_select", C.c_int32), ("done", C.c_int32),
+                ("busy_s", C.c_double)]
 
 
 
@@ -103,6 +104,7 @@ _SIGNATURES = {
                                      C.c_void_p]),
     "pycd_batch_rows": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
     "pycd_batch_msd": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pycd_batch_kernel_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "pycd_batch_add_family": (C.c_int, [_P, C.POINTER(FamilyC),
                                         C.POINTER(C.c_int32)]),
     "pycd_batch_drop_family": (C.c_int, [_P, C.c_int32]),

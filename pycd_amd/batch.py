@@ -159,13 +159,21 @@ class KmcBatch:
                                               C.byref(ms)), "pycd_batch_run")
         return ms.value
 
+    def kernel_ms(self):
+        """Device time of the KMC kernel alone in the last eager run."""
+        ms = C.c_double(0.0)
+        _check(self.ctx, lib().pycd_batch_kernel_ms(self.h, C.byref(ms)),
+               "pycd_batch_kernel_ms")
+        return ms.value
+
     def results(self):
         res = (TrajResultC * self.n_traj)()
         drift = np.empty((self.n_traj, MAX_CARRIERS, 3))
         _check(self.ctx, lib().pycd_batch_results(self.h, res, _ptr(drift)),
                "pycd_batch_results")
         out = {k: np.array([getattr(r, k) for r in res])
-               for k in ("t", "energy", "hops", "start", "no_select", "done")}
+               for k in ("t", "energy", "hops", "start", "no_select", "done",
+                         "busy_s")}
         out["drift"] = drift
         return out
 

@@ -17,6 +17,7 @@ struct pycd_ctx {
   hipStream_t stream = nullptr;
   std::string error;
   int cu_count = 0;
+  double wall_clock_hz = 1e8;  // rate of the kernels' wall_clock64()
 };
 
 namespace pycd {

@@ -21,6 +21,11 @@ extern "C" int pycd_ctx_create(int device, pycd_ctx** out) {
     return PYCD_ERR_HIP;
   }
   ctx->cu_count = prop.multiProcessorCount;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) ==
+          hipSuccess &&
+      khz > 0)
+    ctx->wall_clock_hz = 1e3 * khz;
   *out = ctx;
   return PYCD_OK;
 }

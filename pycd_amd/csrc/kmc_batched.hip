@@ -92,7 +92,7 @@ struct TrajOut {
   int32_t done;      // 1: the trajectory reached max_hops or filled its path
   int32_t started;   // 1: state arrays hold a run in progress
   int32_t pad0;
-  double pad1;
+  double busy;       // wall-clock ticks the trajectory's wavefront spent on it
 };
 
 struct BatchArgs {
@@ -235,15 +235,42 @@ __host__ __device__ inline uint32_t rel_col(uint32_t a, uint32_t b,
   return uint32_t(((dx * g.ny + dy) * g.nz + dz) * g.nu) + (b >> 24);
 }
 
+// The family's arrays are reached through pointers read from memory, which
+// the compiler would treat as generic (flat) addresses; flat loads also
+// count against the LDS wait counter, so every LDS wait would wait for the
+// gathers as well.  These casts make them global loads.
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ inline gptr<T> as_global(const T* p) {
+  return (gptr<T>)p;
+}
+
+__device__ inline SubRec load_rec(const FamDev& f, uint32_t k) {
+  const gptr<SubRec> p = as_global(f.subs) + k;
+  SubRec r;
+  r.tgt = p->tgt;
+  r.pad0 = 0;
+  r.lam = p->lam;
+  r.vab = p->vab;
+  r.tself = p->tself;
+  r.dphi = p->dphi;
+  r.dE = p->dE;
+  r.vec[0] = p->vec[0];
+  r.vec[1] = p->vec[1];
+  r.vec[2] = p->vec[2];
+  return r;
+}
+
 template <typename TT>
-__device__ inline const TT* crow(const FamDev& f, uint32_t sub);
+__device__ inline gptr<TT> crow(const FamDev& f, uint32_t sub);
 template <>
-__device__ inline const double* crow<double>(const FamDev& f, uint32_t sub) {
-  return f.C64 + int64_t(sub) * f.N;
+__device__ inline gptr<double> crow<double>(const FamDev& f, uint32_t sub) {
+  return as_global(f.C64) + int64_t(sub) * f.N;
 }
 template <>
-__device__ inline const float* crow<float>(const FamDev& f, uint32_t sub) {
-  return f.C32 + int64_t(sub) * f.N;
+__device__ inline gptr<float> crow<float>(const FamDev& f, uint32_t sub) {
+  return as_global(f.C32) + int64_t(sub) * f.N;
 }
 
 // A[a, b] for packed sites a, b.
@@ -261,7 +288,7 @@ __device__ inline void psi_from_table(const FamDev& f, const Geo& g, double q,
                                       int lane) {
   for (int r = lane; r < R; r += 64) {
     const uint32_t a = sites[r];
-    const TT* row = crow<TT>(f, a >> 24);
+    const gptr<TT> row = crow<TT>(f, a >> 24);
     double acc = 0.0;
 #pragma unroll 4
     for (int c2 = 0; c2 < nc; ++c2)
@@ -356,6 +383,7 @@ kmc_batched_kernel(BatchArgs a) {
     const TrajOut o0 = a.out[tr];
     if (uni(o0.done)) continue;
     const bool resume = uni(o0.started) != 0;
+    const uint64_t w_start = wall_clock64();
     const TrajDev td = a.trajs[tr];
     const FamDev f = a.fams[uni(td.fam)];
     const Geo g{f.nx, f.ny, f.nz, f.nu};
@@ -386,9 +414,9 @@ kmc_batched_kernel(BatchArgs a) {
       const uint32_t s =
           resume ? socc[c] : site_pack(uint32_t(a.occ0[td.occ_off + c]), g);
       occ[c] = s;
-      cperm[c] = f.perm[site_index(s, g)];
+      cperm[c] = as_global(f.perm)[site_index(s, g)];
     }
-    for (int k = lane; k < f.nu * z; k += 64) tg[k] = f.subs[k].tgt;
+    for (int k = lane; k < f.nu * z; k += 64) tg[k] = as_global(f.subs)[k].tgt;
     if (!resume)
       for (int k = lane; k < w; k += 64) {
         sdisp[k] = 0.0;
@@ -408,8 +436,9 @@ kmc_batched_kernel(BatchArgs a) {
       const int c = p / z, j = p - c * z;
       const uint32_t s = occ[c];
       const uint32_t k = (s >> 24) * z + ((cperm[c] >> (3 * j)) & 7u);
-      fill_process<kF32Rate>(pcd, pcf, p, f.subs[k], q, td.field, field_on);
-      pperm[p] = f.perm[site_index(site_add(s, tg[k], g), g)];
+      fill_process<kF32Rate>(pcd, pcf, p, load_rec(f, k), q, td.field,
+                             field_on);
+      pperm[p] = as_global(f.perm)[site_index(site_add(s, tg[k], g), g)];
     }
     __syncthreads();
     double t_clk, energy;
@@ -431,7 +460,7 @@ kmc_batched_kernel(BatchArgs a) {
         double inner = 0.0;
         for (int c2 = 0; c2 < nc; ++c2)
           inner = fma(q, pair_entry<TT>(f, g, s, occ[c2]), inner);
-        e_loc += q * (2.0 * f.phiL[s >> 24] + inner);
+        e_loc += q * (2.0 * as_global(f.phiL)[s >> 24] + inner);
       }
       energy = td.ewald_neut + (f.qLAqL + wave_sum(e_loc));
       t_clk = 0.0;
@@ -442,6 +471,11 @@ kmc_batched_kernel(BatchArgs a) {
     __syncthreads();
     int64_t slice_done = 0;
     bool finished = false;
+    // next hop count that is a positive multiple of `refresh`
+    int64_t next_refresh = INT64_MAX;
+    if (a.refresh > 0)
+      next_refresh =
+          hops == 0 ? a.refresh : (hops + a.refresh - 1) / a.refresh * a.refresh;
     const float kTf = float(td.kT);
     const float nuf = float(a.nu);
 
@@ -452,7 +486,8 @@ kmc_batched_kernel(BatchArgs a) {
       }
       if (a.slice_hops > 0 && slice_done >= a.slice_hops) break;
       ++slice_done;
-      if (a.refresh > 0 && hops > 0 && hops % a.refresh == 0) {
+      if (hops == next_refresh) {
+        next_refresh += a.refresh;
         // bound the rounding carried by the incremental updates
         psi_from_table<TT>(f, g, q, R, nc, sites, occ, psi, lane);
         __syncthreads();
@@ -568,8 +603,8 @@ kmc_batched_kernel(BatchArgs a) {
       uint32_t tperm = 0;
       if (lane < z) {
         const uint32_t k = sub_new * z + ((perm_new >> (3 * lane)) & 7u);
-        rec_new = f.subs[k];
-        tperm = f.perm[site_index(site_add(s_new, tg[k], g), g)];
+        rec_new = load_rec(f, k);
+        tperm = as_global(f.perm)[site_index(site_add(s_new, tg[k], g), g)];
       }
       uint32_t nsel[kZp - 1];  // targets of the new site, in its slot order
 #pragma unroll
@@ -577,8 +612,8 @@ kmc_batched_kernel(BatchArgs a) {
         nsel[k] = k < z ? uni(site_add(
                               s_new, tg[sub_new * z + ((perm_new >> (3 * k)) & 7u)], g))
                         : 0u;
-      const TT* rn = crow<TT>(f, sub_new);
-      const TT* ro = crow<TT>(f, s_old >> 24);
+      const gptr<TT> rn = crow<TT>(f, sub_new);
+      const gptr<TT> ro = crow<TT>(f, s_old >> 24);
       constexpr int kRPer = kMaxCarriers * kZp / 64;
       double dv[kRPer];
 #pragma unroll
@@ -611,7 +646,7 @@ kmc_batched_kernel(BatchArgs a) {
       for (int cc = 0; cc < 3; ++cc)
         if (lane == (3 * m + cc) % 64) own_k = 3 * m + cc;
       const bool owner = own_k >= 0;
-      if (owner) vec_own = f.subs[rec_old].vec[own_k - 3 * m];
+      if (owner) vec_own = as_global(f.subs)[rec_old].vec[own_k - 3 * m];
       // ---- apply: the new processes first (their records arrived first;
       // process m's old entries are dead once it is selected)
       if (lane < z) {
@@ -685,7 +720,7 @@ kmc_batched_kernel(BatchArgs a) {
       o.done = finished ? 1 : 0;
       o.started = 1;
       o.pad0 = 0;
-      o.pad1 = 0.0;
+      o.busy = (resume ? o0.busy : 0.0) + double(wall_clock64() - w_start);
       a.out[tr] = o;
     }
     __syncthreads();
@@ -694,7 +729,13 @@ kmc_batched_kernel(BatchArgs a) {
 
 // Squared displacement per trajectory, averaged over its carriers:
 //   msd[t, tau] = mean_c mean_{t0 < P_t - tau} |r_c(t0 + tau) - r_c(t0)|^2
-// One workgroup per (trajectory, block of 8 lags).
+// One workgroup of 256 per (trajectory, block of kMsdBlock lags).  Each
+// 16-lane row of a wavefront takes one group of 8 lags at a time, its lanes
+// 16 consecutive carriers (so a row's loads are contiguous in the path
+// rows), and walks t0 in chunks of 4 with the 12 positions
+// r(t0 + tau0 .. t0 + tau0 + 11) of the chunk in registers: a chunk loads 8
+// positions and forms 32 squared distances.  The row sums its carriers with
+// DPP and its last lane writes the 8 lags.
 struct MsdTraj {
   int64_t row_off;
   int64_t P;
@@ -704,7 +745,41 @@ struct MsdTraj {
   int64_t M;
 };
 
+constexpr int kMsdBlock = 512;  // lags per workgroup (64 groups of 8)
+constexpr int kMsdChunk = 4;    // t0 per chunk
+
+template <bool kMasked>
+__device__ inline void msd_chunk(const double* wx, const double* wy,
+                                 const double* wz, const double* x0,
+                                 const double* y0, const double* z0,
+                                 int64_t T, int64_t tmax, uint32_t lmask,
+                                 double* acc) {
+#pragma unroll
+  for (int s = 0; s < kMsdChunk; ++s)
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const double dx = wx[s + l] - x0[s], dy = wy[s + l] - y0[s],
+                   dz = wz[s + l] - z0[s];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (kMasked) {
+        if (((lmask >> l) & 1u) && T + s + l < tmax) acc[l] += d2;
+      } else {
+        acc[l] += d2;
+      }
+    }
+}
+
+// Sum over the 16 lanes of each row, left in the row's last lane.
+__device__ inline double row_sum16(double v) {
+  v += dpp_f64<kDppRowShr + 1, 0xf, true>(v);
+  v += dpp_f64<kDppRowShr + 2, 0xf, true>(v);
+  v += dpp_f64<kDppRowShr + 4, 0xf, true>(v);
+  v += dpp_f64<kDppRowShr + 8, 0xf, true>(v);
+  return v;
+}
+
 __global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(2)))
 msd_batched_kernel(const double* __restrict__ rows,
                    const MsdTraj* __restrict__ tr,
                    const TrajOut* __restrict__ outs, int n_traj, int nlb_max,
@@ -712,53 +787,74 @@ msd_batched_kernel(const double* __restrict__ rows,
   const int t = blockIdx.x / nlb_max;
   const int lb = blockIdx.x - t * nlb_max;
   if (t >= n_traj) return;
-  MsdTraj d = tr[t];
+  const MsdTraj d = tr[t];
   // only the rows written by the run: [0, start) (all P when the path filled)
   const int64_t written = outs[d.traj].start;
-  const int64_t M_all = d.M;
-  d.P = written < 1 ? 1 : (written < d.P ? written : d.P);
-  const int64_t tau0 = int64_t(lb) * 8;
-  if (tau0 >= M_all) return;
+  const int64_t P = written < 1 ? 1 : (written < d.P ? written : d.P);
+  const int64_t blk0 = int64_t(lb) * kMsdBlock;
+  if (blk0 >= d.M) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane & 15, rw = lane >> 4;
   const int w = 3 * d.nc;
-  const double* base = rows + d.row_off;
-  double acc[8];
+  constexpr int C = kMsdChunk;
+  // 64 groups of 8 lags: 16 rows (4 per wavefront) x 4 passes
+  for (int pass = 0; pass < 4; ++pass) {
+    const int grp = pass * 16 + wave * 4 + rw;
+    const int64_t tau0 = blk0 + 8 * grp;
+    if (blk0 + 8 * (pass * 16 + wave * 4) >= d.M) break;  // whole wavefront
+    uint32_t lmask = 0;  // lags of this row inside [1, M)
 #pragma unroll
-  for (int l = 0; l < 8; ++l) acc[l] = 0.0;
-  // per lag the carriers are summed with the 1/(P - tau) weight applied at
-  // the end (same for every carrier of the trajectory)
-  for (int64_t idx = threadIdx.x; idx < d.P * d.nc; idx += blockDim.x) {
-    const int64_t t0 = idx / d.nc;
-    const int c = int(idx - t0 * d.nc);
-    const double* p0 = base + t0 * w + 3 * c;
-    const double x0 = p0[0] * scale, y0 = p0[1] * scale, z0 = p0[2] * scale;
+    for (int l = 0; l < 8; ++l)
+      if (tau0 + l >= 1 && tau0 + l < d.M) lmask |= 1u << l;
+    const int64_t tmax = P - tau0;  // t0 < tmax for lag tau0
+    double acc[8];
 #pragma unroll
-    for (int l = 0; l < 8; ++l) {
-      const int64_t tau = tau0 + l;
-      if (tau >= 1 && tau < d.M && t0 + tau < d.P) {
-        const double* p1 = base + (t0 + tau) * w + 3 * c;
-        const double dx = p1[0] * scale - x0, dy = p1[1] * scale - y0,
-                     dz = p1[2] * scale - z0;
-        acc[l] += dx * dx + dy * dy + dz * dz;
+    for (int l = 0; l < 8; ++l) acc[l] = 0.0;
+    if (lmask && tmax > 0) {
+      for (int c = cl; c < d.nc; c += 16) {
+        const double* pc = rows + d.row_off + 3 * c;
+        // position at path step tt (clamped into the written rows)
+        auto load = [&](int64_t tt, double& x, double& y, double& z) {
+          const double* q = pc + (tt < P ? tt : P - 1) * w;
+          x = q[0] * scale;
+          y = q[1] * scale;
+          z = q[2] * scale;
+        };
+        double wx[8 + C], wy[8 + C], wz[8 + C];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) load(tau0 + k, wx[k], wy[k], wz[k]);
+        for (int64_t T = 0; T < tmax; T += C) {
+          double x0[C], y0[C], z0[C];
+#pragma unroll
+          for (int k = 0; k < C; ++k) {
+            load(tau0 + T + 8 + k, wx[8 + k], wy[8 + k], wz[8 + k]);
+            load(T + k, x0[k], y0[k], z0[k]);
+          }
+          if (lmask == 0xffu && T + (C - 1) + 7 < tmax)
+            msd_chunk<false>(wx, wy, wz, x0, y0, z0, T, tmax, lmask, acc);
+          else
+            msd_chunk<true>(wx, wy, wz, x0, y0, z0, T, tmax, lmask, acc);
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            wx[k] = wx[k + C];
+            wy[k] = wy[k + C];
+            wz[k] = wz[k + C];
+          }
+        }
       }
     }
-  }
-  __shared__ double part[8][256];
+    // every lane of the wavefront takes part in the row sums
 #pragma unroll
-  for (int l = 0; l < 8; ++l) part[l][threadIdx.x] = acc[l];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s)
+    for (int l = 0; l < 8; ++l) acc[l] = row_sum16(acc[l]);
+    if (cl == 15) {
 #pragma unroll
-      for (int l = 0; l < 8; ++l) part[l][threadIdx.x] += part[l][threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x < 8) {
-    const int64_t tau = tau0 + threadIdx.x;
-    if (tau < M_all) {
-      double v = 0.0;
-      if (tau >= 1 && tau < d.M && tau < d.P)
-        v = part[threadIdx.x][0] / double(d.P - tau) / double(d.nc);
-      out[d.out_off + tau] = v;
+      for (int l = 0; l < 8; ++l) {
+        const int64_t tau = tau0 + l;
+        if (tau >= d.M) break;
+        double v = 0.0;
+        if (tau >= 1 && tau < P) v = acc[l] / double(P - tau) / double(d.nc);
+        out[d.out_off + tau] = v;
+      }
     }
   }
 }
@@ -868,6 +964,7 @@ struct pycd_batch {
   int grid = 0;
   hipGraphExec_t graph = nullptr;
   float last_ms = 0.f;
+  float kernel_ms = 0.f;  // the KMC kernel alone, last eager run or step
   ~pycd_batch() {
     if (graph) (void)hipGraphExecDestroy(graph);
     for (auto* f : fams) delete f;
