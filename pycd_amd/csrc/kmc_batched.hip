@@ -224,15 +224,17 @@ __host__ __device__ inline uint32_t site_add(uint32_t s, uint32_t d,
 
 // Column of site b in the compact row of a's sublattice: b moved by
 // -cell(a), as a site index.
-__host__ __device__ inline uint32_t rel_col(uint32_t a, uint32_t b,
-                                            const Geo& g) {
-  int dx = int(b & 0xffu) - int(a & 0xffu);
-  dx += dx < 0 ? g.nx : 0;
-  int dy = int((b >> 8) & 0xffu) - int((a >> 8) & 0xffu);
-  dy += dy < 0 ? g.ny : 0;
-  int dz = int((b >> 16) & 0xffu) - int((a >> 16) & 0xffu);
-  dz += dz < 0 ? g.nz : 0;
-  return uint32_t(((dx * g.ny + dy) * g.nz + dz) * g.nu) + (b >> 24);
+// (N < 2^24, so every partial product fits the 24-bit multiplier; the
+// wrapped difference is min(t, t - n) in unsigned arithmetic.)
+__device__ inline uint32_t rel_col(uint32_t a, uint32_t b, const Geo& g) {
+  const uint32_t tx = (b & 0xffu) + uint32_t(g.nx) - (a & 0xffu);
+  const uint32_t ty = ((b >> 8) & 0xffu) + uint32_t(g.ny) - ((a >> 8) & 0xffu);
+  const uint32_t tz = ((b >> 16) & 0xffu) + uint32_t(g.nz) - ((a >> 16) & 0xffu);
+  const uint32_t dx = min(tx, tx - uint32_t(g.nx));
+  const uint32_t dy = min(ty, ty - uint32_t(g.ny));
+  const uint32_t dz = min(tz, tz - uint32_t(g.nz));
+  return __umul24(__umul24(__umul24(dx, g.ny) + dy, g.nz) + dz, g.nu) +
+         (b >> 24);
 }
 
 // The family's arrays are reached through pointers read from memory, which
@@ -302,26 +304,30 @@ __device__ inline void psi_from_table(const FamDev& f, const Geo& g, double q,
 //   mt     624 words (2560 B)        MT19937 block
 //   psi    rmax doubles              potential of the carriers at each slot
 //   pcd    pmax x kPcD doubles       static part of each process's dG0
-//   pcf    pmax x 2 floats           (FP32 rates) lambda, V_ab + field shift
+//   pcf    pmax x 3 floats           (FP32 rates) lambda, 1 / (4 lambda),
+//                                    V_ab + field shift
 //   sites  rmax words                packed slot sites (carrier, targets)
 //   occ    nc_max words              packed carrier sites
 //   cperm  nc_max words              slot permutation of each carrier's site
 //   tg     sub_max words             the family's hop targets (sub, slot)
 //   pperm  pmax words                slot permutation of each process's
 //                                    target site
+//   disp, base, drift  3 nc_max doubles each: displacement since the last
+//                      path row, position at it, field-weighted drift
 template <bool kF32Rate>
 struct PcLayout {
   static constexpr int kPcD = kF32Rate ? 1 : 6;
-  static constexpr int kPcF = kF32Rate ? 2 : 0;
+  static constexpr int kPcF = kF32Rate ? 3 : 0;
 };
 
 __host__ __device__ inline size_t batch_smem_bytes(int nc_max, int z_max,
                                                   bool f32rate, int sub_max) {
   const size_t rmax = size_t(nc_max) * (1 + z_max);
   const size_t pmax = size_t(nc_max) * z_max;
-  const size_t pcd = f32rate ? 1 : 6, pcf = f32rate ? 2 : 0;
+  const size_t pcd = f32rate ? 1 : 6, pcf = f32rate ? 3 : 0;
   return 2560 + 8 * rmax + 8 * pmax * pcd + 4 * pmax * pcf + 4 * rmax +
-         8 * size_t(nc_max) + 4 * size_t(sub_max) + 4 * pmax;
+         8 * size_t(nc_max) + 4 * size_t(sub_max) + 4 * pmax +
+         72 * size_t(nc_max) + 8;
 }
 
 // Static part of process p's dG0 and rate inputs, from its hop record.
@@ -336,8 +342,9 @@ __device__ inline void fill_process(double* pcd, float* pcf, int p,
                : 0.0;
   if (kF32Rate) {
     pcd[p] = 2 * q * (rec.dphi + tq) + rec.dE;
-    pcf[2 * p] = float(rec.lam);
-    pcf[2 * p + 1] = float(rec.vab) + float(shift);
+    pcf[3 * p] = float(rec.lam);
+    pcf[3 * p + 1] = float(0.25 / rec.lam);
+    pcf[3 * p + 2] = float(rec.vab) + float(shift);
   } else {
     double* e = pcd + 6 * p;
     e[0] = rec.dphi;
@@ -373,6 +380,11 @@ kmc_batched_kernel(BatchArgs a) {
   uint32_t* cperm = occ + a.nc_max;
   uint32_t* tg = cperm + a.nc_max;
   uint32_t* pperm = tg + a.sub_max;
+  // 8-byte alignment for the doubles after the word arrays
+  double* ldisp = reinterpret_cast<double*>(
+      (reinterpret_cast<uintptr_t>(pperm + pmax) + 7) & ~uintptr_t(7));
+  double* lbase = ldisp + 3 * a.nc_max;
+  double* ldrift = lbase + 3 * a.nc_max;
 
   for (;;) {
     uint32_t ti = 0;
@@ -386,18 +398,33 @@ kmc_batched_kernel(BatchArgs a) {
     const uint64_t w_start = wall_clock64();
     const TrajDev td = a.trajs[tr];
     const FamDev f = a.fams[uni(td.fam)];
-    const Geo g{f.nx, f.ny, f.nz, f.nu};
-    const int nc = td.nc, z = f.z, zp = z + 1;
+    const Geo g{uni(f.nx), uni(f.ny), uni(f.nz), uni(f.nu)};
+    const int nc = uni(td.nc), z = uni(f.z), zp = z + 1;
     const int R = nc * zp, nproc = nc * z;
     const int per = (nproc + 63) / 64;
+    // this lane's processes p = lane per + i, fixed for the trajectory:
+    // c | j << 6 | (slot of c's site) << 9 | (slot of the target) << 19,
+    // bit 31 set for a real process
+    uint32_t pdesc[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int p = lane * per + i;
+      pdesc[i] = 0;
+      if (i < per && p < nproc) {
+        const int c = p / z, j = p - c * z;
+        pdesc[i] = uint32_t(c) | (uint32_t(j) << 6) |
+                   (uint32_t(c * zp) << 9) | (uint32_t(c * zp + 1 + j) << 19) |
+                   0x80000000u;
+      }
+    }
     const double q = td.q;
     double* rows = a.rows + td.row_off;
     const int w = 3 * nc;
     const bool field_on =
         td.field[0] != 0.0 || td.field[1] != 0.0 || td.field[2] != 0.0;
 
-    // per-trajectory state in global memory: each entry k of disp, base and
-    // drift is read and written only by lane k % 64
+    // resumable per-trajectory state in global memory (read at the start of
+    // a slice, written at its end)
     uint32_t* mts = a.mts + int64_t(tr) * 625;
     uint32_t* socc = a.st_occ + int64_t(tr) * kMaxCarriers;
     double* svec = a.st_vec + int64_t(tr) * kVecStride;
@@ -417,13 +444,12 @@ kmc_batched_kernel(BatchArgs a) {
       cperm[c] = as_global(f.perm)[site_index(s, g)];
     }
     for (int k = lane; k < f.nu * z; k += 64) tg[k] = as_global(f.subs)[k].tgt;
-    if (!resume)
-      for (int k = lane; k < w; k += 64) {
-        sdisp[k] = 0.0;
-        sbase[k] = 0.0;
-        sdrift[k] = 0.0;
-        rows[k] = 0.0;  // path row 0: the initial positions
-      }
+    for (int k = lane; k < w; k += 64) {
+      ldisp[k] = resume ? sdisp[k] : 0.0;
+      lbase[k] = resume ? sbase[k] : 0.0;
+      ldrift[k] = resume ? sdrift[k] : 0.0;
+      if (!resume) rows[k] = 0.0;  // path row 0: the initial positions
+    }
     __syncthreads();
     for (int r = lane; r < R; r += 64) {
       const int c = r / zp, k = r - c * zp;
@@ -476,9 +502,13 @@ kmc_batched_kernel(BatchArgs a) {
     if (a.refresh > 0)
       next_refresh =
           hops == 0 ? a.refresh : (hops + a.refresh - 1) / a.refresh * a.refresh;
-    const float kTf = float(td.kT);
+    const float nikTf = float(-1.0 / td.kT);
     const float nuf = float(a.nu);
 
+    // uniforms of the next hop, drawn ahead while a hop's gathers are in
+    // flight; only committed (mti advanced) when that hop runs
+    bool pre_ok = false;
+    double pre_u1 = 0.0, pre_u2 = 0.0, pre_lu2 = 0.0;
     for (;;) {
       if (td.max_hops > 0 && hops >= td.max_hops) {
         finished = true;
@@ -492,13 +522,20 @@ kmc_batched_kernel(BatchArgs a) {
         psi_from_table<TT>(f, g, q, R, nc, sites, occ, psi, lane);
         __syncthreads();
       }
-      // ---- two uniforms, as random.random() draws them; the clock's
-      // logarithm does not depend on the rates and overlaps them
-      uint32_t wd[4];
-      mt_next4(mt, mti, lane, wd);
-      const double u1 = mt_uniform(wd[0], wd[1]);
-      const double u2 = mt_uniform(wd[2], wd[3]);
-      const double lu2 = u2 > 0.0 ? log(u2) : 0.0;
+      // ---- two uniforms, as random.random() draws them
+      double u1, u2, lu2;
+      if (pre_ok) {
+        u1 = pre_u1;
+        u2 = pre_u2;
+        lu2 = pre_lu2;
+        mti += 4;
+      } else {
+        uint32_t wd[4];
+        mt_next4(mt, mti, lane, wd);
+        u1 = mt_uniform(wd[0], wd[1]);
+        u2 = mt_uniform(wd[2], wd[3]);
+        lu2 = u2 > 0.0 ? log(u2) : 0.0;
+      }
       // ---- rates from shared memory only
       double kv[kPer], gv[kPer];
       double ksum = 0.0;
@@ -506,18 +543,19 @@ kmc_batched_kernel(BatchArgs a) {
       for (int i = 0; i < kPer; ++i) {
         kv[i] = 0.0;
         gv[i] = 0.0;
-        const int p = lane * per + i;
-        if (i < per && p < nproc) {
-          const int c = p / z;
-          const int j = p - c * z;
-          const double dpsi = psi[c * zp + 1 + j] - psi[c * zp];
+        if (i < per) {
+          const uint32_t d = pdesc[i];
+          const bool real = d >> 31;
+          const int p = real ? lane * per + i : 0;
+          const double dpsi =
+              psi[(d >> 19) & 0x3ffu] - psi[(d >> 9) & 0x3ffu];
           double g0, k;
           if (kF32Rate) {
             g0 = pcd[p] + 2 * q * dpsi;
-            const float lam = pcf[2 * p];
+            const float lam = pcf[3 * p];
             const float gf = float(g0);
-            const float gs = (lam + gf) * (lam + gf) / (4.f * lam) - pcf[2 * p + 1];
-            k = double(nuf * __expf(-gs / kTf));
+            const float gs = (lam + gf) * (lam + gf) * pcf[3 * p + 1] - pcf[3 * p + 2];
+            k = double(nuf * __expf(gs * nikTf));
           } else {
             const double* e = pcd + 6 * p;
             g0 = 2 * q * ((e[0] + dpsi) + e[1]) + e[2];
@@ -526,33 +564,46 @@ kmc_batched_kernel(BatchArgs a) {
                 (((lam + g0) * (lam + g0) / (4 * lam)) - e[4]) - e[5];
             k = a.nu * pow(kEulerE, -gs / td.kT);
           }
-          kv[i] = k;
+          kv[i] = real ? k : 0.0;
           gv[i] = g0;
-          ksum += k;
+          ksum += kv[i];
         }
       }
-      const double k_tot = wave_sum(ksum);
-      // ---- FP64 prefix of k / k_tot and selection
+      // ---- FP64 prefix and selection.  FP64 rates: the prefix of the
+      // quotients k / k_tot against u1, as the reference forms it; FP32
+      // rates: one scan of the rates themselves against u1 k_tot.
+      double k_tot, thr, cum;
       double xq[kPer];
-      double xs = 0.0;
+      if (kF32Rate) {
+        const double incl = wave_incl_scan(ksum);
+        k_tot = read_lane(incl, 63);
+        cum = dpp_f64<kDppWaveShr1, 0xf, true>(incl);
+        thr = u1 * k_tot;
 #pragma unroll
-      for (int i = 0; i < kPer; ++i) {
-        xq[i] = kv[i] / k_tot;
-        xs += xq[i];
+        for (int i = 0; i < kPer; ++i) xq[i] = kv[i];
+      } else {
+        k_tot = wave_sum(ksum);
+        double xs = 0.0;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+          xq[i] = kv[i] / k_tot;
+          xs += xq[i];
+        }
+        cum = wave_excl_scan(xs);
+        thr = u1;
       }
-      double cum = wave_excl_scan(xs);
       int found = -1;
+      uint32_t d_found = 0;
       double g_found = 0.0, k_found = 0.0;
 #pragma unroll
       for (int i = 0; i < kPer; ++i) {
-        const int p = lane * per + i;
-        if (i < per && p < nproc) {
+        if (i < per) {
           cum += xq[i];
-          if (found < 0 && cum > u1) {
-            found = p;
-            g_found = gv[i];
-            k_found = kv[i];
-          }
+          const bool hit = found < 0 && (pdesc[i] >> 31) && cum > thr;
+          found = hit ? lane * per + i : found;
+          d_found = hit ? pdesc[i] : d_found;
+          g_found = hit ? gv[i] : g_found;
+          k_found = hit ? kv[i] : k_found;
         }
       }
       const unsigned long long mask = __ballot(found >= 0);
@@ -568,6 +619,7 @@ kmc_batched_kernel(BatchArgs a) {
 #pragma unroll
           for (int ii = 0; ii < kPer; ++ii)
             if (ii == i) {
+              d_found = pdesc[ii];
               g_found = gv[ii];
               k_found = kv[ii];
             }
@@ -576,6 +628,7 @@ kmc_batched_kernel(BatchArgs a) {
       }
       src = uni(src);
       const int sel = __builtin_amdgcn_readlane(found, src);
+      const uint32_t dsel = uint32_t(__builtin_amdgcn_readlane(int(d_found), src));
       const double gsel = read_lane(g_found, src);
       const double ksel = read_lane(k_found, src);
       // ---- clock
@@ -590,8 +643,8 @@ kmc_batched_kernel(BatchArgs a) {
       }
       energy += gsel;
       // ---- one round of independent loads for the whole update
-      const int m = sel / z;
-      const int j = sel - m * z;
+      const int m = int(dsel & 63u);
+      const int j = int((dsel >> 6) & 7u);
       const uint32_t s_old = uni(sites[m * zp]);
       const uint32_t s_new = uni(sites[m * zp + 1 + j]);
       const uint32_t sub_new = s_new >> 24;
@@ -639,14 +692,18 @@ kmc_batched_kernel(BatchArgs a) {
             part[k] = q * pair_entry<TT>(f, g, nsel[k], o);
         }
       }
-      // displacement by the owner lanes of carrier m's three components
-      double vec_own = 0.0;
-      int own_k = -1;  // the component of carrier m this lane owns, if any
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc)
-        if (lane == (3 * m + cc) % 64) own_k = 3 * m + cc;
-      const bool owner = own_k >= 0;
-      if (owner) vec_own = as_global(f.subs)[rec_old].vec[own_k - 3 * m];
+      // hop vector of carrier m, one component per lane 0..2
+      const double vec_own = lane < 3 ? as_global(f.subs)[rec_old].vec[lane] : 0.0;
+      // next hop's uniforms while the loads are in flight (not across a
+      // regeneration of the MT block)
+      pre_ok = mti <= kMtN - 4;
+      if (pre_ok) {
+        const uint32_t w0 = mt_temper(mt[mti]), w1 = mt_temper(mt[mti + 1]);
+        const uint32_t w2 = mt_temper(mt[mti + 2]), w3 = mt_temper(mt[mti + 3]);
+        pre_u1 = mt_uniform(w0, w1);
+        pre_u2 = mt_uniform(w2, w3);
+        pre_lu2 = pre_u2 > 0.0 ? log(pre_u2) : 0.0;
+      }
       // ---- apply: the new processes first (their records arrived first;
       // process m's old entries are dead once it is selected)
       if (lane < z) {
@@ -659,9 +716,9 @@ kmc_batched_kernel(BatchArgs a) {
         const int r = lane + 64 * i;
         if (r < R) psi[r] += q * dv[i];
       }
-      if (owner) {
-        sdisp[own_k] += vec_own;
-        if (field_on) sdrift[own_k] += vec_own * ksel;
+      if (lane < 3) {
+        ldisp[3 * m + lane] += vec_own;
+        if (field_on) ldrift[3 * m + lane] += vec_own * ksel;
       }
       __syncthreads();
       const double v_new = uni(psi[m * zp + 1 + j]);
@@ -689,10 +746,10 @@ kmc_batched_kernel(BatchArgs a) {
       if (end >= start + 1) {
         if (end >= td.P) end = td.P;
         for (int k = lane; k < w; k += 64) {
-          const double v = sbase[k] + sdisp[k];
+          const double v = lbase[k] + ldisp[k];
           for (int64_t r = start; r < end; ++r) rows[r * w + k] = v;
-          sbase[k] = v;
-          sdisp[k] = 0.0;
+          lbase[k] = v;
+          ldisp[k] = 0.0;
         }
         start = end;
         full = end >= td.P;
@@ -704,8 +761,12 @@ kmc_batched_kernel(BatchArgs a) {
       }
     }
     // ---- outputs and resumable state
-    for (int k = lane; k < w; k += 64)
-      a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = sdrift[k];
+    for (int k = lane; k < w; k += 64) {
+      a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = ldrift[k];
+      sdisp[k] = ldisp[k];
+      sbase[k] = lbase[k];
+      sdrift[k] = ldrift[k];
+    }
     for (int k = lane; k < 624; k += 64) mts[k] = mt[k];
     for (int c = lane; c < nc; c += 64) socc[c] = occ[c];
     for (int r = lane; r < R; r += 64) spsi[r] = psi[r];
