@@ -124,14 +124,19 @@ def run_batched(args, log):
     bt.run(use_graph=False)
     bt_k = bt.kernel_ms()
     busy = res["busy_s"]
+    resident = min(wpc, bt.resident_per_cu())
+    from pycd_amd import _native
+    cus = _native.device_cus()
     log(f"  kernel {bt_k:.1f} ms; longest trajectory {busy.max() * 1e3:.1f} ms; "
-        f"busy sum / resident waves {busy.sum() / (wpc * 256) * 1e3:.1f} ms")
+        f"busy sum / resident waves ({resident}/CU) "
+        f"{busy.sum() / (resident * cus) * 1e3:.1f} ms")
     algo = algorithmic_bytes_batched(trajs, fams, hops, res["start"], 8)
     achieved = algo / (bt_k / 1e3) / 1e9
     info = {
         "ms_per_step": ms, "total_hops": total_hops, "hops_per_s": value,
         "waves_per_cu": wpc, "no_select": int(res["no_select"].sum()),
         "longest_trajectory_ms": float(busy.max() * 1e3),
+        "resident_waves_per_cu": resident,
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -192,7 +197,7 @@ def run_serve(args, log):
 
     t0 = time.time()
     srv = KmcServer(family, n_slots=2048, nc_max=32, z_max=2,
-                    rows_capacity=2048 * 401 * 96, slice_hops=args.serve_slice,
+                    rows_capacity=2048 * 801 * 96, slice_hops=args.serve_slice,
                     rate_precision="f32", path_rows=200, msd_lags=100,
                     waves_per_cu=8)
     for edge in range(4, 13):
@@ -325,6 +330,9 @@ def main():
         "p50_latency_ms": serve["p50_latency_ms"] if serve else None,
         "p99_latency_ms": serve["p99_latency_ms"] if serve else None,
         "serve": serve,
+        "total_hops": info["total_hops"],
+        "longest_trajectory_ms": info["longest_trajectory_ms"],
+        "resident_waves_per_cu": info["resident_waves_per_cu"],
         "roofline": info["roofline"],
         "cpu_baseline": cpu,
     }

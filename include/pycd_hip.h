@@ -56,6 +56,8 @@ typedef struct pycd_ctx pycd_ctx;
 int pycd_ctx_create(int device, pycd_ctx** out);
 void pycd_ctx_destroy(pycd_ctx* ctx);
 const char* pycd_last_error(const pycd_ctx* ctx);
+/* Compute units of the context's device. */
+int pycd_ctx_cu_count(const pycd_ctx* ctx);
 /* Symbol present so that loaders can check the ABI revision. */
 int pycd_abi_version(void);
 
@@ -307,6 +309,9 @@ int pycd_batch_slot_msd(pycd_batch* b, const int64_t* slots, int64_t n,
 /* Device time of the KMC kernel alone (no MSD, no resets) in the last
  * pycd_batch_run without a graph, or the last pycd_batch_step. */
 int pycd_batch_kernel_ms(const pycd_batch* b, double* ms);
+/* Wavefronts of the KMC kernel that one CU holds at once for the current
+ * set-up (registers and LDS); a grid beyond that many per CU queues. */
+int pycd_batch_resident(const pycd_batch* b, int32_t* per_cu);
 
 /* Per-trajectory results and drift sums (n_traj x 64 x 3, carrier-major). */
 int pycd_batch_results(pycd_batch* b, pycd_kmc_traj_result* results,

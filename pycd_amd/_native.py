@@ -75,6 +75,7 @@ _SIGNATURES = {
     "pycd_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "pycd_ctx_destroy": (None, [_P]),
     "pycd_last_error": (C.c_char_p, [_P]),
+    "pycd_ctx_cu_count": (C.c_int, [_P]),
     "pycd_pair_table": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P, C.c_double,
                                   C.c_double, C.c_double, C.c_double,
                                   C.c_int32, _P, C.POINTER(EwaldReport)]),
@@ -105,6 +106,7 @@ _SIGNATURES = {
     "pycd_batch_rows": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
     "pycd_batch_msd": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pycd_batch_kernel_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "pycd_batch_resident": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "pycd_batch_add_family": (C.c_int, [_P, C.POINTER(FamilyC),
                                         C.POINTER(C.c_int32)]),
     "pycd_batch_drop_family": (C.c_int, [_P, C.c_int32]),
@@ -164,6 +166,11 @@ def context(device=None):
                               f"usable HIP device")
         _ctx[device] = h
     return _ctx[device]
+
+
+def device_cus(device=None):
+    """Compute units of the device behind `context(device)`."""
+    return int(lib().pycd_ctx_cu_count(context(device)))
 
 
 def _ptr(a):

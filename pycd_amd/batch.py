@@ -166,6 +166,13 @@ class KmcBatch:
                "pycd_batch_kernel_ms")
         return ms.value
 
+    def resident_per_cu(self):
+        """Wavefronts of the KMC kernel one CU holds for this set-up."""
+        n = C.c_int32(0)
+        _check(self.ctx, lib().pycd_batch_resident(self.h, C.byref(n)),
+               "pycd_batch_resident")
+        return n.value
+
     def results(self):
         res = (TrajResultC * self.n_traj)()
         drift = np.empty((self.n_traj, MAX_CARRIERS, 3))

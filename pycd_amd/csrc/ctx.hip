@@ -42,6 +42,10 @@ extern "C" void pycd_ctx_destroy(pycd_ctx* ctx) {
   delete ctx;
 }
 
+extern "C" int pycd_ctx_cu_count(const pycd_ctx* ctx) {
+  return ctx ? ctx->cu_count : 0;
+}
+
 extern "C" const char* pycd_last_error(const pycd_ctx* ctx) {
   return ctx ? ctx->error.c_str() : "null context";
 }

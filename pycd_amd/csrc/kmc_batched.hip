@@ -363,9 +363,7 @@ __device__ inline void fill_process(double* pcd, float* pcf, int p,
 #define PYCD_BATCH_WAVES_PER_EU 2
 #endif
 template <typename TT, bool kF32Rate, int kPer, int kZp>
-__global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(PYCD_BATCH_WAVES_PER_EU)))
-kmc_batched_kernel(BatchArgs a) {
+__device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
   using L = PcLayout<kF32Rate>;
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x;
@@ -786,6 +784,22 @@ kmc_batched_kernel(BatchArgs a) {
     }
     __syncthreads();
   }
+}
+
+template <typename TT, bool kF32Rate, int kPer, int kZp>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(PYCD_BATCH_WAVES_PER_EU)))
+kmc_batched_kernel(BatchArgs a) {
+  kmc_batched_body<TT, kF32Rate, kPer, kZp>(a);
+}
+
+// The serving pool's slices: the same body under its own name, so that a
+// profile tells the batch launch from the pool's launches.
+template <typename TT, bool kF32Rate, int kPer, int kZp>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(PYCD_BATCH_WAVES_PER_EU)))
+kmc_pool_kernel(BatchArgs a) {
+  kmc_batched_body<TT, kF32Rate, kPer, kZp>(a);
 }
 
 // Squared displacement per trajectory, averaged over its carriers:

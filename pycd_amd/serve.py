@@ -17,6 +17,14 @@ one `pycd_batch` and loops:
 The kernel saves each trajectory's complete state between launches, so a
 job's result is the same whatever else shares its launches and however it
 is sliced (tests/test_gpu_serve.py checks this bit for bit).
+
+Failure handling, all from the host between launches (every kernel exits
+normally): `cancel(job)` stops a pending or running job and frees its slot
+and rows; `cancel(job, requeue=True)` starts it over from its seed; a
+launch whose device time exceeds `watchdog_ms` is treated as a timed-out
+slice, and every job in it is re-queued from the start.  A job's stream
+and time grid are fixed by its parameters, so a re-run gives the same
+result.
 """
 import ctypes as C
 import time
@@ -88,11 +96,15 @@ class KmcServer:
     def __init__(self, family_builder, n_slots=2048, nc_max=32, z_max=4,
                  rows_capacity=1 << 28, table_budget_bytes=32 << 30,
                  slice_hops=2000, rate_precision="f32", path_rows=200,
-                 msd_lags=100, waves_per_cu=8, nu=None):
+                 msd_lags=100, waves_per_cu=8, nu=None, watchdog_ms=None):
         """family_builder(edge) -> (Run, table) for that lattice size."""
         self.build_family = family_builder
         self.slice_hops = int(slice_hops)
         self.path_rows = int(path_rows)
+        # a job's time grid spans ~4x the estimated duration of its hop
+        # budget (the zero-interaction rate estimate runs fast for many
+        # carriers, and a full path ends the job)
+        self.n_path = 4 * self.path_rows + 1
         self.msd_lags = int(msd_lags)
         self.table_budget = int(table_budget_bytes)
         self.nc_max = int(nc_max)
@@ -114,6 +126,9 @@ class KmcServer:
         self.rejected = []
         self.launch_ms = []
         self.evictions = 0
+        self.watchdog_ms = watchdog_ms
+        self.timeouts = 0
+        self.requeued = 0
 
     # -- device objects ----------------------------------------------------
     def _ensure_batch(self, nu):
@@ -191,7 +206,7 @@ class KmcServer:
         """Reason the job can never be admitted, or None."""
         if job.n_carriers > self.nc_max:
             return f"{job.n_carriers} carriers > pool limit {self.nc_max}"
-        if (2 * self.path_rows + 1) * 3 * job.n_carriers > self.rows.capacity:
+        if self.n_path * 3 * job.n_carriers > self.rows.capacity:
             return "path rows exceed the arena"
         if job.edge in self.families:
             return None
@@ -208,7 +223,7 @@ class KmcServer:
     def _try_admit(self, job):
         if not self.free_slots:
             return False
-        n_path = 2 * self.path_rows + 1
+        n_path = self.n_path
         need = n_path * 3 * job.n_carriers
         at = self.rows.alloc(need)
         if at is None:
@@ -272,6 +287,31 @@ class KmcServer:
             admitted += 1
         return admitted
 
+    def cancel(self, job, requeue=False):
+        """Stop a pending or running job and free its slot and rows.  With
+        `requeue` it goes back to the head of the queue and starts over;
+        otherwise its result is {"error": "cancelled"}."""
+        if job in self.pending:
+            self.pending.remove(job)
+        elif job.slot is not None and self.active.get(job.slot) is job:
+            s = job.slot
+            del self.active[s]
+            _check(self.ctx, lib().pycd_batch_release(self.h, s),
+                   "pycd_batch_release")
+            self.rows.release(job.rows_at, job.n_path * 3 * job.n_carriers)
+            self.free_slots.append(s)
+        else:
+            return False
+        job.slot = job.rows_at = None
+        if requeue:
+            self.requeued += 1
+            self.pending.appendleft(job)
+        else:
+            job.result = {"error": "cancelled", "hops": 0}
+            job.finished = time.perf_counter()
+            self.rejected.append(job)
+        return True
+
     def step(self):
         """One slice over every admitted job; retire the finished ones."""
         if not self.active:
@@ -281,6 +321,14 @@ class KmcServer:
                                                C.byref(ms)),
                "pycd_batch_step")
         self.launch_ms.append(ms.value)
+        if self.watchdog_ms is not None and ms.value > self.watchdog_ms:
+            # a timed-out slice: its jobs start over, newest first so that
+            # the queue keeps their arrival order
+            self.timeouts += 1
+            for job in sorted(self.active.values(),
+                              key=lambda j: -j.arrival):
+                self.cancel(job, requeue=True)
+            return []
         res = (TrajResultC * self.n_slots)()
         _check(self.ctx, lib().pycd_batch_results(self.h, res, None),
                "pycd_batch_results")
@@ -350,7 +398,8 @@ class KmcServer:
                 "mean_latency_ms": float(lat.mean() * 1e3),
                 "launches": len(self.launch_ms),
                 "device_ms": float(np.sum(self.launch_ms)),
-                "table_evictions": self.evictions}
+                "table_evictions": self.evictions,
+                "timeouts": self.timeouts, "requeued": self.requeued}
 
 
 def poisson_jobs(n_jobs, rate=200.0, edges=(4, 12), nc=(1, 32),

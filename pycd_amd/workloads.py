@@ -196,7 +196,7 @@ def monoclinic(size, **kw):
 def batched_config(n_traj=4096, hops=100_000, seed=2, param_seed=1234,
                    edges=(4, 5, 6, 7, 8), nc_range=(1, 64),
                    field_max_vpm=1e7, t_range=(250.0, 400.0),
-                   path_rows=1000, msd_lags=500):
+                   path_rows=1000, msd_lags=500, pilot_hops=3000):
     """The batched workload: n_traj trajectories of uneven size and load.
 
     Edge n ~ U{edges}, carriers ~ U{nc_range}, |E| ~ U(0, field_max) V/m
@@ -205,7 +205,12 @@ def batched_config(n_traj=4096, hops=100_000, seed=2, param_seed=1234,
     trajectory parameters come from a NumPy generator seeded `param_seed`;
     the KMC streams follow the reference's seeding protocol from `seed`.
     The path-step width is chosen per trajectory so that `hops` hops span
-    about `path_rows` rows (2 x path_rows rows are allocated).
+    about `path_rows` rows (2 x path_rows + 1 rows are allocated), from the
+    clock of a `pilot_hops`-hop run of the same trajectory on the GPU (the
+    zero-interaction rate estimate alone is several times too fast for
+    many carriers, which would end trajectories early on a full path);
+    `pilot_hops=0` keeps the estimate.  Every trajectory then runs its
+    `hops` hops (the configuration's step count) unless the path fills.
     """
     from . import batch as B
     from .rng import seeded_starts
@@ -240,4 +245,17 @@ def batched_config(n_traj=4096, hops=100_000, seed=2, param_seed=1234,
                           carrier_q=-1.0, field=field, kT=kT,
                           time_interval=dt, ewald_neut=0.0,
                           n_path=2 * path_rows + 1, max_hops=hops))
+    if pilot_hops > 0:
+        bt = B.KmcBatch(fams, table_precision="f64", rate_precision="f32")
+        try:
+            pilot = [dict(t, max_hops=pilot_hops) for t in trajs]
+            bt.setup(pilot, occs, words, waves_per_cu=8)
+            bt.run()
+            res = bt.results()
+        finally:
+            bt.close()
+        for k, t in enumerate(trajs):
+            h, tt = float(res["hops"][k]), float(res["t"][k])
+            if h > 0 and np.isfinite(tt) and tt > 0:
+                t["time_interval"] = hops * (tt / h) / path_rows
     return fams, works, trajs, occs, words, [msd_lags] * n_traj

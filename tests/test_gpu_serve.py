@@ -106,3 +106,44 @@ def test_realtime_poisson_small():
     print(stats)
     assert stats["jobs"] == 30
     assert stats["p99_latency_ms"] >= stats["p50_latency_ms"] > 0
+
+
+def test_cancel_and_timeout_requeue_rerun_identically():
+    """Failures forced from the host between launches: a cancelled job
+    frees its slot; a re-queued job and the jobs of a slice the watchdog
+    declares timed out start over and end with the results of an
+    undisturbed run."""
+    kw = dict(n_slots=8, nc_max=8, rows_capacity=1 << 20, slice_hops=400,
+              rate_precision="f32", path_rows=30, msd_lags=10)
+    specs = [dict(edge=3, n_carriers=3, hops=2000, seed=11),
+             dict(edge=4, n_carriers=5, hops=1500, seed=12),
+             dict(edge=3, n_carriers=2, hops=2500, seed=13)]
+    clean = KmcServer(corundum_family, **kw)
+    ref = [Job(k, **sp) for k, sp in enumerate(specs)]
+    for j in ref:
+        clean.submit(j)
+    clean.drain()
+    clean.close()
+
+    srv = KmcServer(corundum_family, **kw)
+    jobs = [Job(k, **sp) for k, sp in enumerate(specs)]
+    extra = Job(9, edge=3, n_carriers=4, hops=5000, seed=99)
+    for j in jobs + [extra]:
+        srv.submit(j)
+    srv.admit_pending()
+    srv.step()
+    free_before = len(srv.free_slots)
+    assert srv.cancel(extra)
+    assert extra.result["error"] == "cancelled"
+    assert len(srv.free_slots) == free_before + 1
+    assert srv.cancel(jobs[1], requeue=True)
+    srv.watchdog_ms = 0.0          # the next slice counts as timed out
+    srv.admit_pending()
+    srv.step()
+    assert srv.timeouts == 1 and not srv.active
+    srv.watchdog_ms = None
+    srv.drain()
+    assert len(srv.free_slots) == kw["n_slots"]
+    srv.close()
+    for a, b in zip(ref, jobs):
+        assert _result(a) == _result(b)
