@@ -292,9 +292,11 @@ int pycd_batch_drop_family(pycd_batch* b, int32_t index);
  * every admitted, unfinished slot by at most slice_hops hops in one launch
  * (state is saved on the device between steps, so a trajectory's result
  * does not depend on how it is sliced or what else runs); a finished or
- * cancelled slot is handed back with pycd_batch_release. */
+ * cancelled slot is handed back with pycd_batch_release.  sub_max (<= 256)
+ * bounds n_unit z of every family the pool will run (its per-wavefront LDS
+ * holds one entry per sublattice slot). */
 int pycd_batch_pool(pycd_batch* b, int64_t n_slots, int32_t nc_max,
-                    int32_t z_max, int64_t rows_capacity,
+                    int32_t z_max, int32_t sub_max, int64_t rows_capacity,
                     int32_t waves_per_cu);
 int pycd_batch_admit(pycd_batch* b, int64_t slot, const pycd_kmc_traj* traj,
                      const int64_t* occupancy, const uint32_t* mt_state,

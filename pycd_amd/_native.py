@@ -111,7 +111,7 @@ _SIGNATURES = {
                                         C.POINTER(C.c_int32)]),
     "pycd_batch_drop_family": (C.c_int, [_P, C.c_int32]),
     "pycd_batch_pool": (C.c_int, [_P, C.c_int64, C.c_int32, C.c_int32,
-                                  C.c_int64, C.c_int32]),
+                                  C.c_int32, C.c_int64, C.c_int32]),
     "pycd_batch_admit": (C.c_int, [_P, C.c_int64, C.POINTER(TrajC), _P, _P,
                                    C.c_int64]),
     "pycd_batch_release": (C.c_int, [_P, C.c_int64]),

@@ -43,6 +43,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace pycd {
 
@@ -299,61 +300,73 @@ __device__ inline void psi_from_table(const FamDev& f, const Geo& g, double q,
   }
 }
 
+// Per (sublattice, slot) of the trajectory's family: the static part of a
+// hop's dG0 and its rate inputs for the trajectory's charge and field, and
+// the hop vector and target.  FP32 rates: 2 q (dphi + q tself) + dE, lambda,
+// 1 / (4 lambda), V_ab + field shift; FP64 rates: the six terms as the
+// reference adds them.
+struct PTabF32 {
+  double g0s;
+  float lam, i4l, vs;
+  uint32_t tgt;       // the hop record's packed target offset
+  double vec[3];
+};
+struct PTabF64 {
+  double dphi, tq, dE, lam, vab, shift;
+  double vec[3];
+  uint32_t tgt, pad;
+};
+template <bool kF32Rate>
+using PTab = typename std::conditional<kF32Rate, PTabF32, PTabF64>::type;
+
 // Shared-memory layout of one wavefront (sizes from the batch's maxima:
 // pmax = nc_max z_max processes, rmax = nc_max (1 + z_max) slots):
 //   mt     624 words (2560 B)        MT19937 block
 //   psi    rmax doubles              potential of the carriers at each slot
-//   pcd    pmax x kPcD doubles       static part of each process's dG0
-//   pcf    pmax x 3 floats           (FP32 rates) lambda, 1 / (4 lambda),
-//                                    V_ab + field shift
+//   ptab   sub_max PTab entries      per (sublattice, slot), see PTab
+//   disp   3 nc_max doubles          displacement since the last path row
+//   drift  3 nc_max doubles          field-weighted drift
 //   sites  rmax words                packed slot sites (carrier, targets)
 //   occ    nc_max words              packed carrier sites
 //   cperm  nc_max words              slot permutation of each carrier's site
-//   tg     sub_max words             the family's hop targets (sub, slot)
-//   pperm  pmax words                slot permutation of each process's
-//                                    target site
-//   disp, base, drift  3 nc_max doubles each: displacement since the last
-//                      path row, position at it, field-weighted drift
-template <bool kF32Rate>
-struct PcLayout {
-  static constexpr int kPcD = kF32Rate ? 1 : 6;
-  static constexpr int kPcF = kF32Rate ? 3 : 0;
-};
-
+//   pinfo  pmax words                per process: ptab index << 24 | slot
+//                                    permutation of its target site
 __host__ __device__ inline size_t batch_smem_bytes(int nc_max, int z_max,
                                                   bool f32rate, int sub_max) {
   const size_t rmax = size_t(nc_max) * (1 + z_max);
   const size_t pmax = size_t(nc_max) * z_max;
-  const size_t pcd = f32rate ? 1 : 6, pcf = f32rate ? 3 : 0;
-  return 2560 + 8 * rmax + 8 * pmax * pcd + 4 * pmax * pcf + 4 * rmax +
-         8 * size_t(nc_max) + 4 * size_t(sub_max) + 4 * pmax +
-         72 * size_t(nc_max) + 8;
+  const size_t pt = f32rate ? sizeof(PTabF32) : sizeof(PTabF64);
+  return 2560 + 8 * rmax + pt * size_t(sub_max) + 48 * size_t(nc_max) +
+         4 * rmax + 8 * size_t(nc_max) + 4 * pmax;
 }
 
-// Static part of process p's dG0 and rate inputs, from its hop record.
 template <bool kF32Rate>
-__device__ inline void fill_process(double* pcd, float* pcf, int p,
-                                    const SubRec& rec, double q,
-                                    const double* field, bool field_on) {
+__device__ inline void fill_ptab(PTab<kF32Rate>& e, const SubRec& rec,
+                                 double q, const double* field,
+                                 bool field_on) {
   const double tq = q * rec.tself;
   const double shift =
       field_on ? 0.5 * (field[0] * rec.vec[0] + field[1] * rec.vec[1] +
                         field[2] * rec.vec[2])
                : 0.0;
-  if (kF32Rate) {
-    pcd[p] = 2 * q * (rec.dphi + tq) + rec.dE;
-    pcf[3 * p] = float(rec.lam);
-    pcf[3 * p + 1] = float(0.25 / rec.lam);
-    pcf[3 * p + 2] = float(rec.vab) + float(shift);
+  if constexpr (kF32Rate) {
+    e.g0s = 2 * q * (rec.dphi + tq) + rec.dE;
+    e.lam = float(rec.lam);
+    e.i4l = float(0.25 / rec.lam);
+    e.vs = float(rec.vab) + float(shift);
   } else {
-    double* e = pcd + 6 * p;
-    e[0] = rec.dphi;
-    e[1] = tq;
-    e[2] = rec.dE;
-    e[3] = rec.lam;
-    e[4] = rec.vab;
-    e[5] = shift;
+    e.pad = 0;
+    e.dphi = rec.dphi;
+    e.tq = tq;
+    e.dE = rec.dE;
+    e.lam = rec.lam;
+    e.vab = rec.vab;
+    e.shift = shift;
   }
+  e.vec[0] = rec.vec[0];
+  e.vec[1] = rec.vec[1];
+  e.vec[2] = rec.vec[2];
+  e.tgt = rec.tgt;
 }
 
 // kPer: processes per lane (n_c z <= 64 kPer); kZp: slots per carrier
@@ -364,25 +377,19 @@ __device__ inline void fill_process(double* pcd, float* pcf, int p,
 #endif
 template <typename TT, bool kF32Rate, int kPer, int kZp>
 __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
-  using L = PcLayout<kF32Rate>;
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x;
   const int rmax = a.nc_max * (1 + a.z_max);
   const int pmax = a.nc_max * a.z_max;
   uint32_t* mt = reinterpret_cast<uint32_t*>(smem_raw);
   double* psi = reinterpret_cast<double*>(smem_raw + 2560);
-  double* pcd = psi + rmax;
-  float* pcf = reinterpret_cast<float*>(pcd + size_t(pmax) * L::kPcD);
-  uint32_t* sites = reinterpret_cast<uint32_t*>(pcf + size_t(pmax) * L::kPcF);
+  PTab<kF32Rate>* ptab = reinterpret_cast<PTab<kF32Rate>*>(psi + rmax);
+  double* ldisp = reinterpret_cast<double*>(ptab + a.sub_max);
+  double* ldrift = ldisp + 3 * a.nc_max;
+  uint32_t* sites = reinterpret_cast<uint32_t*>(ldrift + 3 * a.nc_max);
   uint32_t* occ = sites + rmax;
   uint32_t* cperm = occ + a.nc_max;
-  uint32_t* tg = cperm + a.nc_max;
-  uint32_t* pperm = tg + a.sub_max;
-  // 8-byte alignment for the doubles after the word arrays
-  double* ldisp = reinterpret_cast<double*>(
-      (reinterpret_cast<uintptr_t>(pperm + pmax) + 7) & ~uintptr_t(7));
-  double* lbase = ldisp + 3 * a.nc_max;
-  double* ldrift = lbase + 3 * a.nc_max;
+  uint32_t* pinfo = cperm + a.nc_max;
 
   for (;;) {
     uint32_t ti = 0;
@@ -441,28 +448,31 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       occ[c] = s;
       cperm[c] = as_global(f.perm)[site_index(s, g)];
     }
-    for (int k = lane; k < f.nu * z; k += 64) tg[k] = as_global(f.subs)[k].tgt;
+    for (int k = lane; k < f.nu * z; k += 64) {
+      fill_ptab<kF32Rate>(ptab[k], load_rec(f, k), q, td.field, field_on);
+    }
     for (int k = lane; k < w; k += 64) {
       ldisp[k] = resume ? sdisp[k] : 0.0;
-      lbase[k] = resume ? sbase[k] : 0.0;
       ldrift[k] = resume ? sdrift[k] : 0.0;
-      if (!resume) rows[k] = 0.0;  // path row 0: the initial positions
+      if (!resume) {
+        sbase[k] = 0.0;
+        rows[k] = 0.0;  // path row 0: the initial positions
+      }
     }
     __syncthreads();
     for (int r = lane; r < R; r += 64) {
       const int c = r / zp, k = r - c * zp;
       const uint32_t s = occ[c];
       const uint32_t jj = (cperm[c] >> (3 * (k - 1))) & 7u;
-      sites[r] = k == 0 ? s : site_add(s, tg[(s >> 24) * z + jj], g);
+      sites[r] = k == 0 ? s : site_add(s, ptab[(s >> 24) * z + jj].tgt, g);
     }
-    // process cache and the slot permutations of the targets
+    // each process's table entry and the slot permutation of its target
     for (int p = lane; p < nproc; p += 64) {
       const int c = p / z, j = p - c * z;
       const uint32_t s = occ[c];
       const uint32_t k = (s >> 24) * z + ((cperm[c] >> (3 * j)) & 7u);
-      fill_process<kF32Rate>(pcd, pcf, p, load_rec(f, k), q, td.field,
-                             field_on);
-      pperm[p] = as_global(f.perm)[site_index(site_add(s, tg[k], g), g)];
+      pinfo[p] = (k << 24) |
+                 as_global(f.perm)[site_index(site_add(s, ptab[k].tgt, g), g)];
     }
     __syncthreads();
     double t_clk, energy;
@@ -545,21 +555,20 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
           const uint32_t d = pdesc[i];
           const bool real = d >> 31;
           const int p = real ? lane * per + i : 0;
+          const PTab<kF32Rate>& e = ptab[pinfo[p] >> 24];
           const double dpsi =
               psi[(d >> 19) & 0x3ffu] - psi[(d >> 9) & 0x3ffu];
           double g0, k;
-          if (kF32Rate) {
-            g0 = pcd[p] + 2 * q * dpsi;
-            const float lam = pcf[3 * p];
+          if constexpr (kF32Rate) {
+            g0 = e.g0s + 2 * q * dpsi;
             const float gf = float(g0);
-            const float gs = (lam + gf) * (lam + gf) * pcf[3 * p + 1] - pcf[3 * p + 2];
+            const float gs = (e.lam + gf) * (e.lam + gf) * e.i4l - e.vs;
             k = double(nuf * __expf(gs * nikTf));
           } else {
-            const double* e = pcd + 6 * p;
-            g0 = 2 * q * ((e[0] + dpsi) + e[1]) + e[2];
-            const double lam = e[3];
+            g0 = 2 * q * ((e.dphi + dpsi) + e.tq) + e.dE;
+            const double lam = e.lam;
             const double gs =
-                (((lam + g0) * (lam + g0) / (4 * lam)) - e[4]) - e[5];
+                (((lam + g0) * (lam + g0) / (4 * lam)) - e.vab) - e.shift;
             k = a.nu * pow(kEulerE, -gs / td.kT);
           }
           kv[i] = real ? k : 0.0;
@@ -646,22 +655,22 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       const uint32_t s_old = uni(sites[m * zp]);
       const uint32_t s_new = uni(sites[m * zp + 1 + j]);
       const uint32_t sub_new = s_new >> 24;
-      const uint32_t perm_new = uni(pperm[sel]);
-      const uint32_t rec_old = (s_old >> 24) * z + ((uni(cperm[m]) >> (3 * j)) & 7u);
-      // hop records of the new site and the slot permutations of its
-      // targets (lanes < z), issued first: the next process cache
-      SubRec rec_new;
-      uint32_t tperm = 0;
+      const uint32_t info_sel = uni(pinfo[sel]);
+      const uint32_t perm_new = info_sel & 0xffffffu;
+      const uint32_t rec_old = info_sel >> 24;
+      // table entries of the new site's processes and the slot
+      // permutations of their targets (lanes < z), issued first
+      uint32_t info_new = 0;
       if (lane < z) {
         const uint32_t k = sub_new * z + ((perm_new >> (3 * lane)) & 7u);
-        rec_new = load_rec(f, k);
-        tperm = as_global(f.perm)[site_index(site_add(s_new, tg[k], g), g)];
+        info_new = (k << 24) |
+                   as_global(f.perm)[site_index(site_add(s_new, ptab[k].tgt, g), g)];
       }
       uint32_t nsel[kZp - 1];  // targets of the new site, in its slot order
 #pragma unroll
       for (int k = 0; k < kZp - 1; ++k)
         nsel[k] = k < z ? uni(site_add(
-                              s_new, tg[sub_new * z + ((perm_new >> (3 * k)) & 7u)], g))
+                              s_new, ptab[sub_new * z + ((perm_new >> (3 * k)) & 7u)].tgt, g))
                         : 0u;
       const gptr<TT> rn = crow<TT>(f, sub_new);
       const gptr<TT> ro = crow<TT>(f, s_old >> 24);
@@ -691,7 +700,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         }
       }
       // hop vector of carrier m, one component per lane 0..2
-      const double vec_own = lane < 3 ? as_global(f.subs)[rec_old].vec[lane] : 0.0;
+      const double vec_own = lane < 3 ? ptab[rec_old].vec[lane] : 0.0;
       // next hop's uniforms while the loads are in flight (not across a
       // regeneration of the MT block)
       pre_ok = mti <= kMtN - 4;
@@ -702,13 +711,8 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         pre_u2 = mt_uniform(w2, w3);
         pre_lu2 = pre_u2 > 0.0 ? log(pre_u2) : 0.0;
       }
-      // ---- apply: the new processes first (their records arrived first;
-      // process m's old entries are dead once it is selected)
-      if (lane < z) {
-        fill_process<kF32Rate>(pcd, pcf, m * z + lane, rec_new, q, td.field,
-                               field_on);
-        pperm[m * z + lane] = tperm;
-      }
+      // ---- apply (process m's old entries are dead once it is selected)
+      if (lane < z) pinfo[m * z + lane] = info_new;
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
         const int r = lane + 64 * i;
@@ -744,9 +748,9 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       if (end >= start + 1) {
         if (end >= td.P) end = td.P;
         for (int k = lane; k < w; k += 64) {
-          const double v = lbase[k] + ldisp[k];
+          const double v = sbase[k] + ldisp[k];
           for (int64_t r = start; r < end; ++r) rows[r * w + k] = v;
-          lbase[k] = v;
+          sbase[k] = v;
           ldisp[k] = 0.0;
         }
         start = end;
@@ -762,7 +766,6 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     for (int k = lane; k < w; k += 64) {
       a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = ldrift[k];
       sdisp[k] = ldisp[k];
-      sbase[k] = lbase[k];
       sdrift[k] = ldrift[k];
     }
     for (int k = lane; k < 624; k += 64) mts[k] = mt[k];

@@ -96,8 +96,10 @@ class KmcServer:
     def __init__(self, family_builder, n_slots=2048, nc_max=32, z_max=4,
                  rows_capacity=1 << 28, table_budget_bytes=32 << 30,
                  slice_hops=2000, rate_precision="f32", path_rows=200,
-                 msd_lags=100, waves_per_cu=8, nu=None, watchdog_ms=None):
-        """family_builder(edge) -> (Run, table) for that lattice size."""
+                 msd_lags=100, waves_per_cu=8, nu=None, watchdog_ms=None,
+                 sub_max=64):
+        """family_builder(edge) -> (Run, table) for that lattice size.
+        sub_max bounds sites-per-cell x hop slots of the families served."""
         self.build_family = family_builder
         self.slice_hops = int(slice_hops)
         self.path_rows = int(path_rows)
@@ -116,6 +118,7 @@ class KmcServer:
         self.h = None
         self.n_slots = int(n_slots)
         self.z_max = int(z_max)
+        self.sub_max = int(sub_max)
         self.rows = RowArena(rows_capacity)
         self.waves_per_cu = int(waves_per_cu)
         self.rate_precision = rate_precision
@@ -141,7 +144,7 @@ class KmcServer:
             "pycd_batch_create")
         self.h = h
         _check(self.ctx, lib().pycd_batch_pool(
-            self.h, self.n_slots, self.nc_max, self.z_max,
+            self.h, self.n_slots, self.nc_max, self.z_max, self.sub_max,
             self.rows.capacity, self.waves_per_cu), "pycd_batch_pool")
 
     def close(self):
