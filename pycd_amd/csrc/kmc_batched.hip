@@ -178,6 +178,12 @@ __device__ inline double read_lane(double v, int lane) {
 }
 
 // Sum over the wavefront, returned in a scalar register.
+__device__ inline double swap_f64(double* p, double v) {
+  return __longlong_as_double(static_cast<long long>(
+      atomicExch(reinterpret_cast<unsigned long long*>(p),
+                 static_cast<unsigned long long>(__double_as_longlong(v)))));
+}
+
 __device__ inline double wave_sum(double v) {
   return read_lane(wave_incl_scan(v), 63);
 }
@@ -324,8 +330,6 @@ using PTab = typename std::conditional<kF32Rate, PTabF32, PTabF64>::type;
 //   mt     624 words (2560 B)        MT19937 block
 //   psi    rmax doubles              potential of the carriers at each slot
 //   ptab   sub_max PTab entries      per (sublattice, slot), see PTab
-//   disp   3 nc_max doubles          displacement since the last path row
-//   drift  3 nc_max doubles          field-weighted drift
 //   sites  rmax words                packed slot sites (carrier, targets)
 //   occ    nc_max words              packed carrier sites
 //   cperm  nc_max words              slot permutation of each carrier's site
@@ -336,8 +340,8 @@ __host__ __device__ inline size_t batch_smem_bytes(int nc_max, int z_max,
   const size_t rmax = size_t(nc_max) * (1 + z_max);
   const size_t pmax = size_t(nc_max) * z_max;
   const size_t pt = f32rate ? sizeof(PTabF32) : sizeof(PTabF64);
-  return 2560 + 8 * rmax + pt * size_t(sub_max) + 48 * size_t(nc_max) +
-         4 * rmax + 8 * size_t(nc_max) + 4 * pmax;
+  return 2560 + 8 * rmax + pt * size_t(sub_max) + 4 * rmax +
+         8 * size_t(nc_max) + 4 * pmax;
 }
 
 template <bool kF32Rate>
@@ -372,8 +376,12 @@ __device__ inline void fill_ptab(PTab<kF32Rate>& e, const SubRec& rec,
 // kPer: processes per lane (n_c z <= 64 kPer); kZp: slots per carrier
 // (z + 1 <= kZp).  Instantiated for the batch's maxima so that the per-lane
 // arrays stay small and several wavefronts fit per SIMD.
+// Wavefronts per SIMD the register allocation targets: 4 (16 per CU, every
+// trajectory of a 4096-trajectory batch resident at once, at the price of a
+// few spilled registers) for FP32 rates with at most 4 processes per lane;
+// 2 otherwise (the FP64-rate and large instances would spill heavily).
 #ifndef PYCD_BATCH_WAVES_PER_EU
-#define PYCD_BATCH_WAVES_PER_EU 2
+#define PYCD_BATCH_WAVES_PER_EU(f32, per) ((f32) && (per) <= 4 ? 4 : 2)
 #endif
 template <typename TT, bool kF32Rate, int kPer, int kZp>
 __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
@@ -384,9 +392,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
   uint32_t* mt = reinterpret_cast<uint32_t*>(smem_raw);
   double* psi = reinterpret_cast<double*>(smem_raw + 2560);
   PTab<kF32Rate>* ptab = reinterpret_cast<PTab<kF32Rate>*>(psi + rmax);
-  double* ldisp = reinterpret_cast<double*>(ptab + a.sub_max);
-  double* ldrift = ldisp + 3 * a.nc_max;
-  uint32_t* sites = reinterpret_cast<uint32_t*>(ldrift + 3 * a.nc_max);
+  uint32_t* sites = reinterpret_cast<uint32_t*>(ptab + a.sub_max);
   uint32_t* occ = sites + rmax;
   uint32_t* cperm = occ + a.nc_max;
   uint32_t* pinfo = cperm + a.nc_max;
@@ -428,8 +434,10 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     const bool field_on =
         td.field[0] != 0.0 || td.field[1] != 0.0 || td.field[2] != 0.0;
 
-    // resumable per-trajectory state in global memory (read at the start of
-    // a slice, written at its end)
+    // resumable per-trajectory state in global memory.  Displacement and
+    // drift are accumulated there by atomic adds that nothing waits for
+    // (lanes 0-2, one hop after another) and read back by atomic exchange
+    // or add, which see every earlier add of this wavefront at L2.
     uint32_t* mts = a.mts + int64_t(tr) * 625;
     uint32_t* socc = a.st_occ + int64_t(tr) * kMaxCarriers;
     double* svec = a.st_vec + int64_t(tr) * kVecStride;
@@ -451,14 +459,13 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     for (int k = lane; k < f.nu * z; k += 64) {
       fill_ptab<kF32Rate>(ptab[k], load_rec(f, k), q, td.field, field_on);
     }
-    for (int k = lane; k < w; k += 64) {
-      ldisp[k] = resume ? sdisp[k] : 0.0;
-      ldrift[k] = resume ? sdrift[k] : 0.0;
-      if (!resume) {
+    if (!resume)
+      for (int k = lane; k < w; k += 64) {
+        swap_f64(sdisp + k, 0.0);
+        swap_f64(sdrift + k, 0.0);
         sbase[k] = 0.0;
         rows[k] = 0.0;  // path row 0: the initial positions
       }
-    }
     __syncthreads();
     for (int r = lane; r < R; r += 64) {
       const int c = r / zp, k = r - c * zp;
@@ -719,8 +726,8 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         if (r < R) psi[r] += q * dv[i];
       }
       if (lane < 3) {
-        ldisp[3 * m + lane] += vec_own;
-        if (field_on) ldrift[3 * m + lane] += vec_own * ksel;
+        atomicAdd(sdisp + 3 * m + lane, vec_own);
+        if (field_on) atomicAdd(sdrift + 3 * m + lane, vec_own * ksel);
       }
       __syncthreads();
       const double v_new = uni(psi[m * zp + 1 + j]);
@@ -748,10 +755,9 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       if (end >= start + 1) {
         if (end >= td.P) end = td.P;
         for (int k = lane; k < w; k += 64) {
-          const double v = sbase[k] + ldisp[k];
+          const double v = sbase[k] + swap_f64(sdisp + k, 0.0);
           for (int64_t r = start; r < end; ++r) rows[r * w + k] = v;
           sbase[k] = v;
-          ldisp[k] = 0.0;
         }
         start = end;
         full = end >= td.P;
@@ -763,11 +769,8 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       }
     }
     // ---- outputs and resumable state
-    for (int k = lane; k < w; k += 64) {
-      a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = ldrift[k];
-      sdisp[k] = ldisp[k];
-      sdrift[k] = ldrift[k];
-    }
+    for (int k = lane; k < w; k += 64)
+      a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = atomicAdd(sdrift + k, 0.0);
     for (int k = lane; k < 624; k += 64) mts[k] = mt[k];
     for (int c = lane; c < nc; c += 64) socc[c] = occ[c];
     for (int r = lane; r < R; r += 64) spsi[r] = psi[r];
@@ -791,7 +794,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
 
 template <typename TT, bool kF32Rate, int kPer, int kZp>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(PYCD_BATCH_WAVES_PER_EU)))
+__attribute__((amdgpu_waves_per_eu(PYCD_BATCH_WAVES_PER_EU(kF32Rate, kPer))))
 kmc_batched_kernel(BatchArgs a) {
   kmc_batched_body<TT, kF32Rate, kPer, kZp>(a);
 }
@@ -800,7 +803,7 @@ kmc_batched_kernel(BatchArgs a) {
 // profile tells the batch launch from the pool's launches.
 template <typename TT, bool kF32Rate, int kPer, int kZp>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(PYCD_BATCH_WAVES_PER_EU)))
+__attribute__((amdgpu_waves_per_eu(PYCD_BATCH_WAVES_PER_EU(kF32Rate, kPer))))
 kmc_pool_kernel(BatchArgs a) {
   kmc_batched_body<TT, kF32Rate, kPer, kZp>(a);
 }
