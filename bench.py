@@ -124,6 +124,11 @@ def run_batched(args, log):
     bt.run(use_graph=False)
     bt_k = bt.kernel_ms()
     busy = res["busy_s"]
+    ncs = np.array([t["n_carriers"] for t in trajs])
+    for lo, hi in ((1, 16), (17, 32), (33, 48), (49, 64)):
+        sel = (ncs >= lo) & (ncs <= hi)
+        log(f"  n_c {lo:2d}-{hi:2d}: busy mean {busy[sel].mean() * 1e3:.1f} ms,"
+            f" max {busy[sel].max() * 1e3:.1f} ms")
     resident = min(wpc, bt.resident_per_cu())
     from pycd_amd import _native
     cus = _native.device_cus()
