@@ -91,7 +91,8 @@ def measured_traffic(path):
         text = Path(path).read_text()
     except OSError:
         return None
-    m = re.search(r"traffic per launch = .*? = ([0-9.e+]+) B", text)
+    m = re.search(r"traffic per launch(?: \(grid \d+\))? = .*? = ([0-9.e+]+) B",
+                  text)
     return float(m.group(1)) if m else None
 
 
