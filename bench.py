@@ -221,6 +221,90 @@ def run_serve(args, log):
     return stats
 
 
+def run_precision(args, log):
+    """Precision form: 10x10x10 corundum (12000 sites), reciprocal sum with
+    |k| < 2 pi 10 / L_a, table in FP64 / FP32-MFMA / BF16x3-MFMA; per-entry
+    and total-energy error against FP64; then 1e6 hops with 32 electrons on
+    each table (plain path), compared with the FP64-table run."""
+    from pycd_amd import workloads, _native, rng, constants
+    from pycd_amd.rng import seeded_starts
+    n = args.precision_edge
+    w = workloads.corundum(n, m_kcut=n, n_carriers=32, t_final=1e-8)
+    pos = w.neighbors.bulk_sites.cell_coordinates
+    out = {"n_sites": int(w.n_sites)}
+    tables = {}
+    for mode in ("f64", "f32", "bf16x3"):
+        best = None
+        for rep_i in range(2):
+            tab, rep = _native.pair_table(
+                pos, w.neighbors.simulation_cell_matrix,
+                w.system.translational_matrix, w.system.pbc, w.system.alpha,
+                w.system.r_cut, w.system.k_cut, 1.0, mode)
+            if best is None or rep["ms_recip"] < best["ms_recip"]:
+                best = rep
+        tables[mode] = tab
+        # the symmetric product: N (N + 1) / 2 entries of 2 Kp flops each
+        kp = 2 * best["n_kvec"]
+        nn = w.n_sites
+        flops = float(nn) * (nn + 1) * kp * (3 if mode == "bf16x3" else 1)
+        peak = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}[mode]
+        tf = flops / (best["ms_recip"] / 1e3) / 1e12
+        out[mode] = {"ms_sf": best["ms_sf"], "ms_contraction": best["ms_recip"],
+                     "n_kvec_half": best["n_kvec"], "k_max": best["k_max"],
+                     "tflops": tf, "peak_tflops": peak, "frac": tf / peak}
+        log(f"{mode}: {best['ms_recip']:.2f} ms contraction, {tf:.1f} TF/s")
+    ref = tables["f64"]
+    scale = float(np.max(np.abs(ref)))
+    run = w.make_run(ref)
+    occs, words = seeded_starts(2, 1, [run.hop_site_lists["electron"]], [32])
+    q = run.base_charge_config()[:, 0].copy()
+    q[occs[0]] -= 1.0
+    e_ref = float(q @ ref @ q)
+    for mode in ("f32", "bf16x3"):
+        d = tables[mode] - ref
+        out[mode]["max_abs_err"] = float(np.max(np.abs(d)))
+        out[mode]["rms_err"] = float(np.sqrt(np.mean(d * d)))
+        out[mode]["rel_max_err"] = out[mode]["max_abs_err"] / scale
+        out[mode]["energy_err_hartree"] = float(abs(q @ d @ q))
+        out[mode]["energy_rel_err"] = out[mode]["energy_err_hartree"] / abs(e_ref)
+        del d
+    out["energy_f64_hartree"] = e_ref
+    state = (3, tuple(int(v) for v in words[0]), None)
+    hops = args.precision_hops
+    uni = rng.UniformStream(state).take(2 * hops)
+    dev = _native.DeviceArray.from_host(uni)
+    runs = {}
+    for mode in ("f64", "f32", "bf16x3"):
+        pb = _native.KmcProblem(
+            table=tables[mode], lattice_q=run.base_charge_config()[:, 0],
+            nbr_count=run.nbr_count, nbr_site=run.nbr_site,
+            nbr_vec=run.nbr_vec, nbr_lambda=run.nbr_lambda,
+            nbr_vab=run.nbr_vab, site_energy=run.system_relative_energies,
+            carrier_q=np.full(32, -1.0), field=np.zeros(3),
+            nu=run.material.vn, kT=run.temp, time_interval=run.time_interval,
+            n_path=int(run.t_final / run.time_interval) + 1)
+        sess = _native.KmcSession(pb)
+        sess.begin(occs[0])
+        ms = sess.advance_device(dev, hops, max_hops=hops)
+        r = sess.read()
+        sess.close()
+        runs[mode] = r
+        out[mode]["kmc_hops"] = r["hops"]
+        out[mode]["kmc_hops_per_s"] = r["hops"] / (ms / 1e3)
+        out[mode]["kmc_final_time_s"] = r["t"] / constants.SEC2AUTIME
+        log(f"{mode}: {r['hops']} hops, {r['hops'] / (ms / 1e3):.3e} hops/s")
+    for mode in ("f32", "bf16x3"):
+        a, b = runs[mode]["unwrapped"], runs["f64"]["unwrapped"]
+        rows = min(runs[mode]["start"], runs["f64"]["start"])
+        same = np.all(np.abs(a[:rows] - b[:rows]) < 1e-6, axis=1)
+        first = int(np.argmin(same)) if not same.all() else rows
+        out[mode]["path_rows_identical_to_f64"] = first
+        out[mode]["final_time_rel_diff"] = float(
+            abs(runs[mode]["t"] - runs["f64"]["t"]) / runs["f64"]["t"])
+    dev.free()
+    return out
+
+
 def cpu_baseline(args, trajs, fams, log):
     """NumPy oracle on a bounded sample of the batched workload (1 core)."""
     import random

@@ -55,8 +55,9 @@ table_f64(const double* __restrict__ Y, int n, int Kp,
   const int t = threadIdx.x;
   const int tx = t & 15, ty = t >> 4;
   int bi, bj;
-  tile_coords(n, kTile, bi, bj);
+  tile_coords_upper(n, kTile, bi, bj);
   const int i0 = bi * kTile, j0 = bj * kTile;
+  const bool mirror = bi != bj;
 
   double acc[4][4];
 #pragma unroll
@@ -115,7 +116,7 @@ table_f64(const double* __restrict__ Y, int n, int Kp,
     for (int c = 0; c < 4; ++c) {
       const int j = j0 + tx + 16 * c;
       if (j >= n) continue;
-      A[int64_t(i) * n + j] = assemble_entry(i, j, pos, g, es, acc[r][c], cnt);
+      store_pair(A, n, i, j, mirror, pos, g, es, acc[r][c], cnt);
     }
   }
   count_pairs(cnt, pairs);
@@ -148,7 +149,8 @@ int launch_table(pycd_ctx* ctx, int precision, const double* pos, int n,
   }
   PYCD_HIP(ctx, hipEventRecord(after_sf, s));
   const int nb = (n + kTile - 1) / kTile;
-  table_f64<<<nb * nb, 256, 0, s>>>(Y.ptr, n, Kp, pos, geo, es, out, pairs);
+  table_f64<<<nb * (nb + 1) / 2, 256, 0, s>>>(Y.ptr, n, Kp, pos, geo, es, out,
+                                              pairs);
   PYCD_HIP(ctx, hipGetLastError());
   // Y is released when this returns; the stream must have consumed it.
   PYCD_HIP(ctx, hipStreamSynchronize(s));

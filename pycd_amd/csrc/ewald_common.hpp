@@ -69,7 +69,7 @@ inline void fill_mic_geometry(const double* cell, const int32_t* pbc,
     g.dS[k][0] = g.dS[k][1] = g.dS[k][2] = 0.0;
 }
 
-__device__ inline double mic_r2(double dx, double dy, double dz,
+__device__ __forceinline__ double mic_r2(double dx, double dy, double dz,
                                 const MicGeometry& g) {
   double f0 = dx * g.inv[0] + dy * g.inv[3] + dz * g.inv[6];
   double f1 = dx * g.inv[1] + dy * g.inv[4] + dz * g.inv[7];
@@ -93,7 +93,7 @@ __device__ inline double mic_r2(double dx, double dy, double dz,
 
 // Final table entry from the reciprocal accumulator `acc` (= sum over the
 // half space of w cos, i.e. (Y Y^T)[i,j]).
-__device__ inline double assemble_entry(int i, int j,
+__device__ __forceinline__ double assemble_entry(int i, int j,
                                         const double* __restrict__ pos,
                                         const MicGeometry& g,
                                         const EwaldScalars& es, double acc,
@@ -121,26 +121,58 @@ __device__ inline void count_pairs(unsigned cnt,
     atomicAdd(pairs, static_cast<unsigned long long>(cnt));
 }
 
-// Maps the 1-D workgroup index to a (row, column) tile.  Consecutive
-// workgroups are dealt round-robin to the 8 XCDs; the first step gives each
-// XCD a contiguous run of tiles, the second walks them in groups of 8 tile
-// rows so that an XCD's L2 holds the few Y row blocks it is using.
-__device__ inline void tile_coords(int n, int tile, int& bi, int& bj) {
+// The table is symmetric (Y Y^T, and the real-space and self terms), so
+// only tiles with bi <= bj are launched: nb (nb + 1) / 2 workgroups.
+// Consecutive workgroups are dealt round-robin to the 8 XCDs; each XCD
+// walks a contiguous run of tiles.  The run goes through groups of 8 tile
+// rows column by column (each column's rows bi <= bj of the group), so that
+// a column block of Y serves up to 8 consecutive tiles and the group's row
+// blocks stay in cache along the walk.
+__device__ inline void tile_coords_upper(int n, int tile, int& bi, int& bj) {
   const int nb = (n + tile - 1) / tile;
-  const int T = nb * nb;
+  const int T = nb * (nb + 1) / 2;
   int w = blockIdx.x;
   const int full = (T / 8) * 8;
   if (w < full) {
     const int per = full / 8;
     w = (w % 8) * per + (w / 8);
   }
-  const int G = 8;
-  const int group = w / (G * nb);
-  const int first = group * G;
-  const int gsize = min(nb - first, G);
-  const int in = w - group * G * nb;
-  bi = first + in % gsize;
-  bj = in / gsize;
+  constexpr int G = 8;
+  for (int g0 = 0; g0 < nb; g0 += G) {
+    const int gs = min(G, nb - g0);
+    const int head = gs * (gs + 1) / 2;  // columns g0 .. g0 + gs - 1
+    const int count = head + (nb - g0 - gs) * gs;
+    if (w >= count) {
+      w -= count;
+      continue;
+    }
+    if (w < head) {
+      int c = 0;  // column g0 + c holds rows g0 .. g0 + c
+      while (w >= c + 1) {
+        w -= c + 1;
+        ++c;
+      }
+      bj = g0 + c;
+      bi = g0 + w;
+    } else {
+      w -= head;
+      bj = g0 + gs + w / gs;
+      bi = g0 + w % gs;
+    }
+    return;
+  }
+  bi = bj = 0;  // not reached: blockIdx.x < T
+}
+
+// Entry (i, j) of a tile and, off the diagonal block, its mirror (j, i):
+// the mirror's reciprocal sum is the same FP32/FP64 accumulation (the same
+// products in the same order), its real-space part is formed from (j, i).
+__device__ __forceinline__ void store_pair(double* __restrict__ A, int n, int i, int j,
+                                  bool mirror, const double* __restrict__ pos,
+                                  const MicGeometry& g, const EwaldScalars& es,
+                                  double acc, unsigned& cnt) {
+  A[int64_t(i) * n + j] = assemble_entry(i, j, pos, g, es, acc, cnt);
+  if (mirror) A[int64_t(j) * n + i] = assemble_entry(j, i, pos, g, es, acc, cnt);
 }
 
 int launch_table(pycd_ctx* ctx, int precision, const double* pos, int n,

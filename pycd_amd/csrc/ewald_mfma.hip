@@ -12,7 +12,8 @@
 //
 // Tiling: 256 threads = 4 wavefronts per 128 x 128 output tile; each
 // wavefront owns 64 x 64 = 4 x 4 MFMA blocks of 16 x 16.  K is staged
-// through LDS in slices of 32 (row-major, padded rows).
+// through LDS in slices of 32 (row-major, padded rows); the global loads of
+// the next slice are in flight while the current one is multiplied.
 #include "ewald_common.hpp"
 
 #include <hip/hip_bf16.h>
@@ -24,6 +25,20 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr int kMTile = 128;
 constexpr int kMSlice = 32;
+
+// 16-byte load through a global (address-space-1) pointer.  A generic
+// pointer would compile to a flat load, which also counts against the LDS
+// wait counter and would make every LDS read wait for the prefetch.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ inline uint4 load16_global(const void* p) {
+  const u32x4 v = *(const __attribute__((address_space(1))) u32x4*)(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline float4 load16f_global(const void* p) {
+  const uint4 v = load16_global(p);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y),
+                     __uint_as_float(v.z), __uint_as_float(v.w));
+}
 
 // Structure factors in reduced precision from the FP64 values.
 __global__ void __launch_bounds__(256)
@@ -64,8 +79,9 @@ table_mfma_f32(const float* __restrict__ Y, int n, int Kp,
   __shared__ float Bs[kMTile][kMSlice + 1];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   int bi, bj;
-  tile_coords(n, kMTile, bi, bj);
+  tile_coords_upper(n, kMTile, bi, bj);
   const int i0 = bi * kMTile, j0 = bj * kMTile;
+  const bool mirror = bi != bj;
   const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
   v4f acc[4][4];
 #pragma unroll
@@ -76,16 +92,19 @@ table_mfma_f32(const float* __restrict__ Y, int n, int Kp,
   const bool a_ok = i0 + lrow < n, b_ok = j0 + lrow < n;
   const float* ya = Y + int64_t(i0 + lrow) * Kp + lk;
   const float* yb = Y + int64_t(j0 + lrow) * Kp + lk;
-  for (int k0 = 0; k0 < Kp; k0 += kMSlice) {
-    float4 va[4], vb[4];
+  // software pipeline: slice k0 + kMSlice is loaded into registers while the
+  // matrix cores work on slice k0 from LDS
+  float4 va[4], vb[4];
+  auto gload = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      va[q] = a_ok ? *reinterpret_cast<const float4*>(ya + k0 + 4 * q)
+      va[q] = a_ok ? load16f_global(ya + k0 + 4 * q)
                    : make_float4(0.f, 0.f, 0.f, 0.f);
-      vb[q] = b_ok ? *reinterpret_cast<const float4*>(yb + k0 + 4 * q)
+      vb[q] = b_ok ? load16f_global(yb + k0 + 4 * q)
                    : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    __syncthreads();
+  };
+  auto sstore = [&]() {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       As[lrow][lk + 4 * q + 0] = va[q].x;
@@ -97,7 +116,13 @@ table_mfma_f32(const float* __restrict__ Y, int n, int Kp,
       Bs[lrow][lk + 4 * q + 2] = vb[q].z;
       Bs[lrow][lk + 4 * q + 3] = vb[q].w;
     }
-    __syncthreads();
+  };
+  gload(0);
+  sstore();
+  __syncthreads();
+  for (int k0 = 0; k0 < Kp; k0 += kMSlice) {
+    const bool more = k0 + kMSlice < Kp;
+    if (more) gload(k0 + kMSlice);
 #pragma unroll
     for (int kk = 0; kk < kMSlice; kk += 4) {
       float a[4], b[4];
@@ -113,8 +138,14 @@ table_mfma_f32(const float* __restrict__ Y, int n, int Kp,
           acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[x], b[y],
                                                            acc[x][y], 0, 0, 0);
     }
+    __syncthreads();  // every wavefront is done with this slice
+    if (more) sstore();
+    __syncthreads();
   }
   unsigned cnt = 0;
+  // entries (i, j), then the mirrored (j, i) of an off-diagonal tile, in two
+  // fully unrolled passes (one pass with both would not unroll, and the
+  // accumulators would leave the registers)
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -127,6 +158,20 @@ table_mfma_f32(const float* __restrict__ Y, int n, int Kp,
           A[int64_t(i) * n + j] =
               assemble_entry(i, j, pos, g, es, double(acc[x][y][r]), cnt);
       }
+  if (mirror) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r;
+          const int j = j0 + wc + 16 * y + (lane & 15);
+          if (i < n && j < n)
+            A[int64_t(j) * n + i] =
+                assemble_entry(j, i, pos, g, es, double(acc[x][y][r]), cnt);
+        }
+  }
   count_pairs(cnt, pairs);
 }
 
@@ -143,8 +188,9 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
   __shared__ __attribute__((aligned(16))) __bf16 Bl[kMTile * kRow];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   int bi, bj;
-  tile_coords(n, kMTile, bi, bj);
+  tile_coords_upper(n, kMTile, bi, bj);
   const int i0 = bi * kMTile, j0 = bj * kMTile;
+  const bool mirror = bi != bj;
   const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
   v4f acc[4][4];
 #pragma unroll
@@ -157,16 +203,19 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
   const int64_t aoff = int64_t(i0 + lrow) * Kp + lk;
   const int64_t boff = int64_t(j0 + lrow) * Kp + lk;
   const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
-  for (int k0 = 0; k0 < Kp; k0 += kMSlice) {
-    uint4 ah[2], al[2], bh[2], bl[2];
+  // software pipeline: slice k0 + kMSlice is loaded into registers while the
+  // matrix cores work on slice k0 from LDS
+  uint4 ah[2], al[2], bh[2], bl[2];
+  auto gload = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      ah[q] = a_ok ? *reinterpret_cast<const uint4*>(Yhi + aoff + k0 + 8 * q) : zero;
-      al[q] = a_ok ? *reinterpret_cast<const uint4*>(Ylo + aoff + k0 + 8 * q) : zero;
-      bh[q] = b_ok ? *reinterpret_cast<const uint4*>(Yhi + boff + k0 + 8 * q) : zero;
-      bl[q] = b_ok ? *reinterpret_cast<const uint4*>(Ylo + boff + k0 + 8 * q) : zero;
+      ah[q] = a_ok ? load16_global(Yhi + aoff + k0 + 8 * q) : zero;
+      al[q] = a_ok ? load16_global(Ylo + aoff + k0 + 8 * q) : zero;
+      bh[q] = b_ok ? load16_global(Yhi + boff + k0 + 8 * q) : zero;
+      bl[q] = b_ok ? load16_global(Ylo + boff + k0 + 8 * q) : zero;
     }
-    __syncthreads();
+  };
+  auto sstore = [&]() {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       *reinterpret_cast<uint4*>(Ah + lrow * kRow + lk + 8 * q) = ah[q];
@@ -174,7 +223,13 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
       *reinterpret_cast<uint4*>(Bh + lrow * kRow + lk + 8 * q) = bh[q];
       *reinterpret_cast<uint4*>(Bl + lrow * kRow + lk + 8 * q) = bl[q];
     }
-    __syncthreads();
+  };
+  gload(0);
+  sstore();
+  __syncthreads();
+  for (int k0 = 0; k0 < Kp; k0 += kMSlice) {
+    const bool more = k0 + kMSlice < Kp;
+    if (more) gload(k0 + kMSlice);
     v8bf a_h[4], a_l[4], b_h[4], b_l[4];
     const int kof = (lane >> 4) * 8;
 #pragma unroll
@@ -197,8 +252,14 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
         acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[x], b_h[y],
                                                             acc[x][y], 0, 0, 0);
       }
+    __syncthreads();  // every wavefront is done with this slice
+    if (more) sstore();
+    __syncthreads();
   }
   unsigned cnt = 0;
+  // entries (i, j), then the mirrored (j, i) of an off-diagonal tile, in two
+  // fully unrolled passes (one pass with both would not unroll, and the
+  // accumulators would leave the registers)
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -211,6 +272,20 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
           A[int64_t(i) * n + j] =
               assemble_entry(i, j, pos, g, es, double(acc[x][y][r]), cnt);
       }
+  if (mirror) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r;
+          const int j = j0 + wc + 16 * y + (lane & 15);
+          if (i < n && j < n)
+            A[int64_t(j) * n + i] =
+                assemble_entry(j, i, pos, g, es, double(acc[x][y][r]), cnt);
+        }
+  }
   count_pairs(cnt, pairs);
 }
 
@@ -243,12 +318,13 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
   }
   PYCD_HIP(ctx, hipEventRecord(after_sf, s));
   const int nb = (n + kMTile - 1) / kMTile;
+  const int tiles = nb * (nb + 1) / 2;
   if (precision == PYCD_PREC_F32)
-    table_mfma_f32<<<nb * nb, 256, 0, s>>>(y32.ptr, n, Kp, pos, geo, es, out,
-                                           pairs);
+    table_mfma_f32<<<tiles, 256, 0, s>>>(y32.ptr, n, Kp, pos, geo, es, out,
+                                         pairs);
   else
-    table_mfma_bf16x3<<<nb * nb, 256, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, pos,
-                                              geo, es, out, pairs);
+    table_mfma_bf16x3<<<tiles, 256, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, pos,
+                                            geo, es, out, pairs);
   PYCD_HIP(ctx, hipGetLastError());
   PYCD_HIP(ctx, hipStreamSynchronize(s));
   return PYCD_OK;
