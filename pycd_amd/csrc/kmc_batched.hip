@@ -193,6 +193,10 @@ __device__ inline double wave_excl_scan(double v) {
 }
 
 // ---- packed sites of a cell-periodic family
+// A site is cx | cz << 8 | cy << 16 | sub << 24: masking with 0x00ff00ff
+// leaves (cx, cy) as two 16-bit lanes, so the wrapped cell difference of x
+// and y takes packed 16-bit integer instructions, and the column a
+// difference names is one 16-bit dot product.
 struct Geo {
   int nx, ny, nz, nu;
 };
@@ -204,14 +208,13 @@ __host__ __device__ inline uint32_t site_pack(uint32_t i, const Geo& g) {
   c /= uint32_t(g.nz);
   const uint32_t cy = c % uint32_t(g.ny);
   const uint32_t cx = c / uint32_t(g.ny);
-  return cx | (cy << 8) | (cz << 16) | (sub << 24);
+  return cx | (cz << 8) | (cy << 16) | (sub << 24);
 }
 
 __host__ __device__ inline uint32_t site_index(uint32_t p, const Geo& g) {
-  return ((((p & 0xffu) * uint32_t(g.ny) + ((p >> 8) & 0xffu)) *
-               uint32_t(g.nz) +
-           ((p >> 16) & 0xffu)) *
-          uint32_t(g.nu)) +
+  const uint32_t cx = p & 0xffu, cz = (p >> 8) & 0xffu,
+                 cy = (p >> 16) & 0xffu;
+  return ((cx * uint32_t(g.ny) + cy) * uint32_t(g.nz) + cz) * uint32_t(g.nu) +
          (p >> 24);
 }
 
@@ -221,27 +224,63 @@ __host__ __device__ inline uint32_t site_add(uint32_t s, uint32_t d,
                                              const Geo& g) {
   int x = int(s & 0xffu) + int(d & 0xffu);
   x -= x >= g.nx ? g.nx : 0;
-  int y = int((s >> 8) & 0xffu) + int((d >> 8) & 0xffu);
-  y -= y >= g.ny ? g.ny : 0;
-  int w = int((s >> 16) & 0xffu) + int((d >> 16) & 0xffu);
+  int w = int((s >> 8) & 0xffu) + int((d >> 8) & 0xffu);
   w -= w >= g.nz ? g.nz : 0;
-  return uint32_t(x) | (uint32_t(y) << 8) | (uint32_t(w) << 16) |
+  int y = int((s >> 16) & 0xffu) + int((d >> 16) & 0xffu);
+  y -= y >= g.ny ? g.ny : 0;
+  return uint32_t(x) | (uint32_t(w) << 8) | (uint32_t(y) << 16) |
          (d & 0xff000000u);
 }
 
-// Column of site b in the compact row of a's sublattice: b moved by
-// -cell(a), as a site index.
-// (N < 2^24, so every partial product fits the 24-bit multiplier; the
-// wrapped difference is min(t, t - n) in unsigned arithmetic.)
-__device__ inline uint32_t rel_col(uint32_t a, uint32_t b, const Geo& g) {
-  const uint32_t tx = (b & 0xffu) + uint32_t(g.nx) - (a & 0xffu);
-  const uint32_t ty = ((b >> 8) & 0xffu) + uint32_t(g.ny) - ((a >> 8) & 0xffu);
-  const uint32_t tz = ((b >> 16) & 0xffu) + uint32_t(g.nz) - ((a >> 16) & 0xffu);
-  const uint32_t dx = min(tx, tx - uint32_t(g.nx));
-  const uint32_t dy = min(ty, ty - uint32_t(g.ny));
-  const uint32_t dz = min(tz, tz - uint32_t(g.nz));
-  return __umul24(__umul24(__umul24(dx, g.ny) + dy, g.nz) + dz, g.nu) +
-         (b >> 24);
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Per-row part of a column lookup: the compact row of a's sublattice, and
+// (n - a) per axis for the wrapped difference.
+struct RowKey {
+  u16x2 kxy;     // (nx - ax, ny - ay)
+  uint32_t kz;   // nz - az
+  uint32_t sub;  // a's sublattice
+};
+
+// Launch-constant packed sizes and strides (N < 2^24 and ny nz nu < 2^16).
+struct GeoPk {
+  u16x2 nxy;     // (nx, ny)
+  u16x2 sxy;     // (ny nz nu, nz nu)
+  uint32_t nz, nu;
+};
+
+__device__ inline GeoPk geo_pk(const Geo& g) {
+  GeoPk p;
+  p.nxy = u16x2{uint16_t(g.nx), uint16_t(g.ny)};
+  p.sxy = u16x2{uint16_t(g.ny * g.nz * g.nu), uint16_t(g.nz * g.nu)};
+  p.nz = uint32_t(g.nz);
+  p.nu = uint32_t(g.nu);
+  return p;
+}
+
+__device__ inline RowKey row_key(uint32_t a, const GeoPk& p) {
+  RowKey k;
+  k.kxy = p.nxy - __builtin_bit_cast(u16x2, a & 0x00ff00ffu);
+  k.kz = p.nz - ((a >> 8) & 0xffu);
+  k.sub = a >> 24;
+  return k;
+}
+
+// Column of site b in the compact row of key's site: b moved by -cell(a),
+// as a site index (the wrapped difference is min(t, t - n) in unsigned
+// arithmetic, t = b + n - a in [1, 2n)).
+__device__ inline uint32_t rel_col(const RowKey& k, uint32_t b,
+                                   const GeoPk& p) {
+  const u16x2 t = __builtin_bit_cast(u16x2, b & 0x00ff00ffu) + k.kxy;
+  const u16x2 dxy = __builtin_elementwise_min(t, t - p.nxy);
+  const uint32_t tz = ((b >> 8) & 0xffu) + k.kz;
+  const uint32_t dz = min(tz, tz - p.nz);
+  return __builtin_amdgcn_udot2(dxy, p.sxy, __umul24(dz, p.nu) + (b >> 24),
+                                false);
+}
+
+__device__ inline uint32_t rel_col(uint32_t a, uint32_t b, const GeoPk& p) {
+  return rel_col(row_key(a, p), b, p);
 }
 
 // The family's arrays are reached through pointers read from memory, which
@@ -284,24 +323,26 @@ __device__ inline gptr<float> crow<float>(const FamDev& f, uint32_t sub) {
 
 // A[a, b] for packed sites a, b.
 template <typename TT>
-__device__ inline double pair_entry(const FamDev& f, const Geo& g, uint32_t a,
-                                    uint32_t b) {
-  return double(crow<TT>(f, a >> 24)[rel_col(a, b, g)]);
+__device__ inline double pair_entry(const FamDev& f, const GeoPk& p,
+                                    uint32_t a, uint32_t b) {
+  return double(crow<TT>(f, a >> 24)[rel_col(a, b, p)]);
 }
 
 // psi[r] = sum_c q A[sites[r], occ[c]] for every slot r (lanes over slots).
 template <typename TT>
-__device__ inline void psi_from_table(const FamDev& f, const Geo& g, double q,
-                                      int R, int nc, const uint32_t* sites,
+__device__ inline void psi_from_table(const FamDev& f, const GeoPk& gp,
+                                      double q, int R, int nc,
+                                      const uint32_t* sites,
                                       const uint32_t* occ, double* psi,
                                       int lane) {
   for (int r = lane; r < R; r += 64) {
     const uint32_t a = sites[r];
-    const gptr<TT> row = crow<TT>(f, a >> 24);
+    const RowKey key = row_key(a, gp);
+    const gptr<TT> row = crow<TT>(f, key.sub);
     double acc = 0.0;
 #pragma unroll 4
     for (int c2 = 0; c2 < nc; ++c2)
-      acc = fma(q, double(row[rel_col(a, occ[c2], g)]), acc);
+      acc = fma(q, double(row[rel_col(key, occ[c2], gp)]), acc);
     psi[r] = acc;
   }
 }
@@ -410,6 +451,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     const TrajDev td = a.trajs[tr];
     const FamDev f = a.fams[uni(td.fam)];
     const Geo g{uni(f.nx), uni(f.ny), uni(f.nz), uni(f.nu)};
+    const GeoPk gp = geo_pk(g);
     const int nc = uni(td.nc), z = uni(f.z), zp = z + 1;
     const int R = nc * zp, nproc = nc * z;
     const int per = (nproc + 63) / 64;
@@ -493,14 +535,14 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       hops = o0.hops;
       no_select = o0.no_select;
     } else {
-      psi_from_table<TT>(f, g, q, R, nc, sites, occ, psi, lane);
+      psi_from_table<TT>(f, gp, q, R, nc, sites, occ, psi, lane);
       // initial energy: ewald_neut + q_L A q_L + sum_c q (2 phi_L + psi_self)
       double e_loc = 0.0;
       for (int c = lane; c < nc; c += 64) {
         const uint32_t s = occ[c];
         double inner = 0.0;
         for (int c2 = 0; c2 < nc; ++c2)
-          inner = fma(q, pair_entry<TT>(f, g, s, occ[c2]), inner);
+          inner = fma(q, pair_entry<TT>(f, gp, s, occ[c2]), inner);
         e_loc += q * (2.0 * as_global(f.phiL)[s >> 24] + inner);
       }
       energy = td.ewald_neut + (f.qLAqL + wave_sum(e_loc));
@@ -534,7 +576,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       if (hops == next_refresh) {
         next_refresh += a.refresh;
         // bound the rounding carried by the incremental updates
-        psi_from_table<TT>(f, g, q, R, nc, sites, occ, psi, lane);
+        psi_from_table<TT>(f, gp, q, R, nc, sites, occ, psi, lane);
         __syncthreads();
       }
       // ---- two uniforms, as random.random() draws them
@@ -681,6 +723,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
                         : 0u;
       const gptr<TT> rn = crow<TT>(f, sub_new);
       const gptr<TT> ro = crow<TT>(f, s_old >> 24);
+      const RowKey key_new = row_key(s_new, gp), key_old = row_key(s_old, gp);
       constexpr int kRPer = kMaxCarriers * kZp / 64;
       double dv[kRPer];
 #pragma unroll
@@ -689,8 +732,8 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         dv[i] = 0.0;
         if (r < R) {
           const uint32_t x = sites[r];
-          dv[i] = double(rn[rel_col(s_new, x, g)]) -
-                  double(ro[rel_col(s_old, x, g)]);
+          dv[i] = double(rn[rel_col(key_new, x, gp)]) -
+                  double(ro[rel_col(key_old, x, gp)]);
         }
       }
       // fresh sums of the new slots (not the one that returns to s_old):
@@ -703,7 +746,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         for (int k = 0; k < kZp - 1; ++k) {
           part[k] = 0.0;
           if (k < z && has && nsel[k] != s_old)
-            part[k] = q * pair_entry<TT>(f, g, nsel[k], o);
+            part[k] = q * pair_entry<TT>(f, gp, nsel[k], o);
         }
       }
       // hop vector of carrier m, one component per lane 0..2
@@ -983,7 +1026,7 @@ compact_check(const double* __restrict__ A, int64_t N, Geo g,
     const double* crow0 = A + int64_t(a >> 24) * N;
     for (int64_t j = lane; j < N; j += 64) {
       const uint32_t b = site_pack(uint32_t(j), g);
-      m = fmax(m, fabs(row[j] - crow0[rel_col(a, b, g)]));
+      m = fmax(m, fabs(row[j] - crow0[rel_col(a, b, geo_pk(g))]));
     }
   }
 #pragma unroll
