@@ -567,6 +567,19 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     bool pre_ok = false;
     double pre_u1 = 0.0, pre_u2 = 0.0, pre_lu2 = 0.0;
     for (;;) {
+      // Conditions on the lane and on the trajectory's sizes are formed
+      // again each hop from these copies, which the compiler cannot move
+      // out of the loop.  Hoisted, each such condition is a 64-bit lane mask
+      // held across the loop, and there are more of them than registers.
+      int lane_h = lane, per_h = per, z_h = z, nc_h = nc, R_h = R;
+      asm volatile("" : "+v"(lane_h));
+      asm volatile("" : "+s"(per_h), "+s"(z_h), "+s"(nc_h), "+s"(R_h));
+      uint32_t pd[kPer];
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) {
+        pd[i] = pdesc[i];
+        asm volatile("" : "+v"(pd[i]));
+      }
       if (td.max_hops > 0 && hops >= td.max_hops) {
         finished = true;
         break;
@@ -576,7 +589,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       if (hops == next_refresh) {
         next_refresh += a.refresh;
         // bound the rounding carried by the incremental updates
-        psi_from_table<TT>(f, gp, q, R, nc, sites, occ, psi, lane);
+        psi_from_table<TT>(f, gp, q, R_h, nc_h, sites, occ, psi, lane_h);
         __syncthreads();
       }
       // ---- two uniforms, as random.random() draws them
@@ -588,7 +601,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         mti += 4;
       } else {
         uint32_t wd[4];
-        mt_next4(mt, mti, lane, wd);
+        mt_next4(mt, mti, lane_h, wd);
         u1 = mt_uniform(wd[0], wd[1]);
         u2 = mt_uniform(wd[2], wd[3]);
         lu2 = u2 > 0.0 ? log(u2) : 0.0;
@@ -600,10 +613,10 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       for (int i = 0; i < kPer; ++i) {
         kv[i] = 0.0;
         gv[i] = 0.0;
-        if (i < per) {
-          const uint32_t d = pdesc[i];
+        if (i < per_h) {
+          const uint32_t d = pd[i];
           const bool real = d >> 31;
-          const int p = real ? lane * per + i : 0;
+          const int p = real ? lane_h * per_h + i : 0;
           const PTab<kF32Rate>& e = ptab[pinfo[p] >> 24];
           const double dpsi =
               psi[(d >> 19) & 0x3ffu] - psi[(d >> 9) & 0x3ffu];
@@ -653,11 +666,11 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       double g_found = 0.0, k_found = 0.0;
 #pragma unroll
       for (int i = 0; i < kPer; ++i) {
-        if (i < per) {
+        if (i < per_h) {
           cum += xq[i];
-          const bool hit = found < 0 && (pdesc[i] >> 31) && cum > thr;
-          found = hit ? lane * per + i : found;
-          d_found = hit ? pdesc[i] : d_found;
+          const bool hit = found < 0 && (pd[i] >> 31) && cum > thr;
+          found = hit ? lane_h * per_h + i : found;
+          d_found = hit ? pd[i] : d_found;
           g_found = hit ? gv[i] : g_found;
           k_found = hit ? kv[i] : k_found;
         }
@@ -668,14 +681,14 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         src = __ffsll(static_cast<long long>(mask)) - 1;
       } else {
         // rounding left every cum <= u1: take the last process
-        src = (nproc - 1) / per;
-        if (lane == src) {
+        src = (nproc - 1) / per_h;
+        if (lane_h == src) {
           found = nproc - 1;
-          const int i = nproc - 1 - lane * per;
+          const int i = nproc - 1 - lane_h * per_h;
 #pragma unroll
           for (int ii = 0; ii < kPer; ++ii)
             if (ii == i) {
-              d_found = pdesc[ii];
+              d_found = pd[ii];
               g_found = gv[ii];
               k_found = kv[ii];
             }
@@ -710,16 +723,16 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       // table entries of the new site's processes and the slot
       // permutations of their targets (lanes < z), issued first
       uint32_t info_new = 0;
-      if (lane < z) {
-        const uint32_t k = sub_new * z + ((perm_new >> (3 * lane)) & 7u);
+      if (lane_h < z_h) {
+        const uint32_t k = sub_new * z_h + ((perm_new >> (3 * lane_h)) & 7u);
         info_new = (k << 24) |
                    as_global(f.perm)[site_index(site_add(s_new, ptab[k].tgt, g), g)];
       }
       uint32_t nsel[kZp - 1];  // targets of the new site, in its slot order
 #pragma unroll
       for (int k = 0; k < kZp - 1; ++k)
-        nsel[k] = k < z ? uni(site_add(
-                              s_new, ptab[sub_new * z + ((perm_new >> (3 * k)) & 7u)].tgt, g))
+        nsel[k] = k < z_h ? uni(site_add(
+                              s_new, ptab[sub_new * z_h + ((perm_new >> (3 * k)) & 7u)].tgt, g))
                         : 0u;
       const gptr<TT> rn = crow<TT>(f, sub_new);
       const gptr<TT> ro = crow<TT>(f, s_old >> 24);
@@ -728,9 +741,9 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       double dv[kRPer];
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
-        const int r = lane + 64 * i;
+        const int r = lane_h + 64 * i;
         dv[i] = 0.0;
-        if (r < R) {
+        if (r < R_h) {
           const uint32_t x = sites[r];
           dv[i] = double(rn[rel_col(key_new, x, gp)]) -
                   double(ro[rel_col(key_old, x, gp)]);
@@ -740,17 +753,17 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       // one carrier per lane (n_c <= 64)
       double part[kZp - 1];
       {
-        const bool has = lane < nc;
-        const uint32_t o = has ? (lane == m ? s_new : occ[lane]) : 0u;
+        const bool has = lane_h < nc_h;
+        const uint32_t o = has ? (lane_h == m ? s_new : occ[lane_h]) : 0u;
 #pragma unroll
         for (int k = 0; k < kZp - 1; ++k) {
           part[k] = 0.0;
-          if (k < z && has && nsel[k] != s_old)
+          if (k < z_h && has && nsel[k] != s_old)
             part[k] = q * pair_entry<TT>(f, gp, nsel[k], o);
         }
       }
       // hop vector of carrier m, one component per lane 0..2
-      const double vec_own = lane < 3 ? ptab[rec_old].vec[lane] : 0.0;
+      const double vec_own = lane_h < 3 ? ptab[rec_old].vec[lane_h] : 0.0;
       // next hop's uniforms while the loads are in flight (not across a
       // regeneration of the MT block)
       pre_ok = mti <= kMtN - 4;
@@ -762,15 +775,15 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         pre_lu2 = pre_u2 > 0.0 ? log(pre_u2) : 0.0;
       }
       // ---- apply (process m's old entries are dead once it is selected)
-      if (lane < z) pinfo[m * z + lane] = info_new;
+      if (lane_h < z_h) pinfo[m * z_h + lane_h] = info_new;
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
-        const int r = lane + 64 * i;
-        if (r < R) psi[r] += q * dv[i];
+        const int r = lane_h + 64 * i;
+        if (r < R_h) psi[r] += q * dv[i];
       }
-      if (lane < 3) {
-        atomicAdd(sdisp + 3 * m + lane, vec_own);
-        if (field_on) atomicAdd(sdrift + 3 * m + lane, vec_own * ksel);
+      if (lane_h < 3) {
+        atomicAdd(sdisp + 3 * m + lane_h, vec_own);
+        if (field_on) atomicAdd(sdrift + 3 * m + lane_h, vec_own * ksel);
       }
       __syncthreads();
       const double v_new = uni(psi[m * zp + 1 + j]);
@@ -778,16 +791,16 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       double fresh_v[kZp - 1];
 #pragma unroll
       for (int k = 0; k < kZp - 1; ++k)
-        fresh_v[k] = (k < z && nsel[k] != s_old) ? wave_sum(part[k]) : v_old;
+        fresh_v[k] = (k < z_h && nsel[k] != s_old) ? wave_sum(part[k]) : v_old;
       __syncthreads();
-      if (lane == 0) {
+      if (lane_h == 0) {
         occ[m] = s_new;
         cperm[m] = perm_new;
         sites[m * zp] = s_new;
         psi[m * zp] = v_new;
 #pragma unroll
         for (int k = 0; k < kZp - 1; ++k)
-          if (k < z) {
+          if (k < z_h) {
             sites[m * zp + 1 + k] = nsel[k];
             psi[m * zp + 1 + k] = fresh_v[k];
           }
@@ -797,7 +810,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       bool full = false;
       if (end >= start + 1) {
         if (end >= td.P) end = td.P;
-        for (int k = lane; k < w; k += 64) {
+        for (int k = lane_h; k < w; k += 64) {
           const double v = sbase[k] + swap_f64(sdisp + k, 0.0);
           for (int64_t r = start; r < end; ++r) rows[r * w + k] = v;
           sbase[k] = v;
