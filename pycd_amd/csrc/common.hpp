@@ -18,6 +18,7 @@ struct pycd_ctx {
   std::string error;
   int cu_count = 0;
   double wall_clock_hz = 1e8;  // rate of the kernels' wall_clock64()
+  std::vector<uint32_t> simd_keys;  // hardware keys of the SIMDs (probed once)
 };
 
 namespace pycd {

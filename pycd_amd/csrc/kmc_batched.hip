@@ -105,7 +105,15 @@ struct BatchArgs {
   double* rows;             // all path rows
   double* drift;            // n_traj x 64 x 3 (nc <= kMaxCarriers)
   TrajOut* out;
+  // work queue: `order` holds n_bins bins back to back, bin b at
+  // [bin_start[b], bin_start[b + 1]); counter[b] counts bin b's pops and
+  // counter[n_bins] all pops.  key_bin maps a SIMD's hardware key to the bin
+  // its wavefronts start from (-1: none; NULL when n_bins is 1).
   uint32_t* counter;
+  const uint32_t* bin_start;
+  const int16_t* key_bin;
+  int32_t n_bins;
+  int32_t pad1;
   // resumable state (slices): MT words, occupancy, displacement/base/drift
   uint32_t* mts;            // n_traj x 625
   uint32_t* st_occ;         // n_traj x kMaxCarriers (packed sites)
@@ -417,6 +425,54 @@ __device__ inline void fill_ptab(PTab<kF32Rate>& e, const SubRec& rec,
 // kPer: processes per lane (n_c z <= 64 kPer); kZp: slots per carrier
 // (z + 1 <= kZp).  Instantiated for the batch's maxima so that the per-lane
 // arrays stay small and several wavefronts fit per SIMD.
+// Where this wavefront runs: XCC << 10 | SE << 7 | SH << 6 | CU << 2 | SIMD
+// (13 bits), from the XCC_ID and HW_ID hardware registers.
+__device__ inline uint32_t simd_key() {
+  // s_getreg operand: (bits - 1) << 11 | offset << 6 | register id
+  const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);   // HW_ID
+  const uint32_t xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20); // XCC_ID
+  const uint32_t simd = (hw >> 4) & 3u, cu = (hw >> 8) & 15u,
+                 sh = (hw >> 12) & 1u, se = (hw >> 13) & 7u;
+  return ((xcc & 7u) << 10) | (se << 7) | (sh << 6) | (cu << 2) | simd;
+}
+constexpr int kSimdKeys = 1 << 13;
+
+// Index into `order` of this wavefront's next trajectory, or n_traj once
+// the queue is empty (lane 0 only).  Pops from `bin` first, then moves on
+// to the next bins; stops scanning as soon as every trajectory is taken.
+__device__ inline uint32_t pop_ticket(const BatchArgs& a, int& bin) {
+  uint32_t* taken = a.counter + a.n_bins;
+  for (int k = 0; k < a.n_bins; ++k) {
+    const uint32_t lo = a.bin_start[bin], n = a.bin_start[bin + 1] - lo;
+    if (n > 0 &&
+        __hip_atomic_load(a.counter + bin, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT) < n) {
+      const uint32_t t = atomicAdd(a.counter + bin, 1u);
+      if (t < n) {
+        atomicAdd(taken, 1u);
+        return lo + t;
+      }
+    }
+    if (__hip_atomic_load(taken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+        uint32_t(a.n_traj))
+      break;
+    bin = bin + 1 == a.n_bins ? 0 : bin + 1;
+  }
+  return uint32_t(a.n_traj);
+}
+
+// Marks the hardware key of every SIMD that runs one of its wavefronts in
+// `seen` (kSimdKeys bits).  Each wavefront sleeps `spin` x ~8k cycles
+// after, so that a grid of several wavefronts per SIMD occupies all of
+// them at once.
+__global__ void __launch_bounds__(64) simd_probe(uint32_t* seen, int spin) {
+  if (threadIdx.x == 0) {
+    const uint32_t k = simd_key();
+    atomicOr(seen + (k >> 5), 1u << (k & 31u));
+  }
+  for (int i = 0; i < spin; ++i) __builtin_amdgcn_s_sleep(127);
+}
+
 // Wavefronts per SIMD the register allocation targets: 4 (16 per CU, every
 // trajectory of a 4096-trajectory batch resident at once, at the price of a
 // few spilled registers) for FP32 rates with at most 4 processes per lane;
@@ -438,10 +494,16 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
   uint32_t* cperm = occ + a.nc_max;
   uint32_t* pinfo = cperm + a.nc_max;
 
+  int bin = 0;
+  if (a.n_bins > 1) {
+    const int b = a.key_bin[simd_key()];
+    bin = uni(b >= 0 ? b : int(blockIdx.x % uint32_t(a.n_bins)));
+  }
   for (;;) {
     uint32_t ti = 0;
-    if (lane == 0) ti = atomicAdd(a.counter, 1u);
+    if (lane == 0) ti = pop_ticket(a, bin);
     ti = uni(uint32_t(__shfl(int(ti), 0, 64)));
+    bin = uni(__shfl(bin, 0, 64));
     if (ti >= uint32_t(a.n_traj)) break;
     const int tr = uni(a.order[ti]);
     const TrajOut o0 = a.out[tr];
@@ -1078,7 +1140,9 @@ struct pycd_batch {
   DevBuf<TrajDev> d_trajs;
   DevBuf<int32_t> d_order;
   DevBuf<int64_t> d_occ;
-  DevBuf<uint32_t> d_mt, d_counter;
+  DevBuf<uint32_t> d_mt, d_counter, d_bins;
+  DevBuf<int16_t> d_keybin;
+  int n_bins = 1;
   DevBuf<double> d_rows, d_drift, d_msd;
   DevBuf<TrajOut> d_out;
   DevBuf<uint32_t> d_mts;

@@ -114,3 +114,33 @@ def test_msd_and_graph_replays():
             assert np.array_equal(a, b)
         bt.run(use_graph=False)
     bt.close()
+
+
+def test_queue_bins_leave_results_unchanged():
+    """Per-SIMD bins (16 waves per CU), per-CU bins (2) and one queue (1)
+    hand trajectories to different wavefronts; every result is the same."""
+    fams, runs = _families([3], t_final=2e-8)
+    rs = np.random.default_rng(5)
+    specs = [(0, int(nc), float(fv), 300.0)
+             for nc, fv in zip(rs.integers(1, 9, 1100),
+                               rs.choice([0.0, 1e6, 5e6], 1100))]
+    trajs, occs, words = _mixed_batch(fams, runs, specs, 17, 120)
+    order = np.argsort([-s[1] for s in specs], kind="stable")
+    out = []
+    for wpc in (16, 2, 1):
+        bt = B.KmcBatch(fams, "f64", "f32")
+        bt.setup(trajs, occs, words, order=order, waves_per_cu=wpc)
+        bt.run()
+        res = bt.results()
+        out.append((res, [bt.rows(k) for k in (0, 1, 777, 1099)]))
+        bt.close()
+    ref, ref_rows = out[0]
+    # every trajectory ran to one of its limits: max_hops or a full path
+    n_path = np.array([t["n_path"] for t in trajs])
+    assert np.all((ref["hops"] == 120) | (ref["start"] == n_path))
+    assert np.all(ref["hops"] > 0)
+    for res, rows in out[1:]:
+        for key in ("t", "hops", "energy", "start", "drift"):
+            assert np.array_equal(res[key], ref[key]), key
+        for a, b in zip(rows, ref_rows):
+            assert np.array_equal(a, b)
