@@ -185,15 +185,60 @@ __device__ inline double read_lane(double v, int lane) {
   return __longlong_as_double((int64_t(hi) << 32) | lo);
 }
 
-// Sum over the wavefront, returned in a scalar register.
 __device__ inline double swap_f64(double* p, double v) {
   return __longlong_as_double(static_cast<long long>(
       atomicExch(reinterpret_cast<unsigned long long*>(p),
                  static_cast<unsigned long long>(__double_as_longlong(v)))));
 }
 
+// Sum over the wavefront, returned in a scalar register.
 __device__ inline double wave_sum(double v) {
   return read_lane(wave_incl_scan(v), 63);
+}
+
+constexpr int kDppRowRor = 0x120;  // + n: rotate each row of 16 by n lanes
+
+__device__ inline uint32_t lo32(double x) {
+  return uint32_t(__double_as_longlong(x));
+}
+__device__ inline uint32_t hi32(double x) {
+  return uint32_t(uint64_t(__double_as_longlong(x)) >> 32);
+}
+__device__ inline double from32(uint32_t lo, uint32_t hi) {
+  return __longlong_as_double(int64_t((uint64_t(hi) << 32) | lo));
+}
+
+// a + b after rows 2-3 of a are exchanged with rows 0-1 of b: rows 0-1
+// hold 32 partial sums of a, rows 2-3 32 partial sums of b.
+__device__ inline double fold32(double a, double b) {
+  const auto l = __builtin_amdgcn_permlane32_swap(lo32(a), lo32(b), false,
+                                                  false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi32(a), hi32(b), false,
+                                                  false);
+  return from32(l[0], h[0]) + from32(l[1], h[1]);
+}
+
+// a + b after the odd rows of a are exchanged with the even rows of b:
+// rows 0 and 2 hold partial sums of a, rows 1 and 3 of b.
+__device__ inline double fold16(double a, double b) {
+  const auto l = __builtin_amdgcn_permlane16_swap(lo32(a), lo32(b), false,
+                                                  false);
+  const auto h = __builtin_amdgcn_permlane16_swap(hi32(a), hi32(b), false,
+                                                  false);
+  return from32(l[0], h[0]) + from32(l[1], h[1]);
+}
+
+// Four sums over the wavefront at once, returned in scalar registers: two
+// folds leave 16 partial sums of v[r] in row r, a rotation butterfly sums
+// each row.  (Four wave_sum calls would take four full reductions.)
+__device__ inline void wave_sum4(const double* v, double* out) {
+  double x = fold16(fold32(v[0], v[2]), fold32(v[1], v[3]));
+  x += dpp_f64<kDppRowRor + 8, 0xf, false>(x);
+  x += dpp_f64<kDppRowRor + 4, 0xf, false>(x);
+  x += dpp_f64<kDppRowRor + 2, 0xf, false>(x);
+  x += dpp_f64<kDppRowRor + 1, 0xf, false>(x);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[r] = read_lane(x, 16 * r);
 }
 
 __device__ inline double wave_excl_scan(double v) {
@@ -813,14 +858,15 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       }
       // fresh sums of the new slots (not the one that returns to s_old):
       // one carrier per lane (n_c <= 64)
-      double part[kZp - 1];
+      constexpr int kFresh = (kZp - 1 + 3) / 4 * 4;
+      double part[kFresh];
       {
         const bool has = lane_h < nc_h;
         const uint32_t o = has ? (lane_h == m ? s_new : occ[lane_h]) : 0u;
 #pragma unroll
-        for (int k = 0; k < kZp - 1; ++k) {
+        for (int k = 0; k < kFresh; ++k) {
           part[k] = 0.0;
-          if (k < z_h && has && nsel[k] != s_old)
+          if (k < kZp - 1 && k < z_h && has && nsel[k] != s_old)
             part[k] = q * pair_entry<TT>(f, gp, nsel[k], o);
         }
       }
@@ -850,10 +896,12 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       __syncthreads();
       const double v_new = uni(psi[m * zp + 1 + j]);
       const double v_old = uni(psi[m * zp]);
-      double fresh_v[kZp - 1];
+      double fresh_v[kFresh];
+#pragma unroll
+      for (int k = 0; k < kFresh; k += 4) wave_sum4(part + k, fresh_v + k);
 #pragma unroll
       for (int k = 0; k < kZp - 1; ++k)
-        fresh_v[k] = (k < z_h && nsel[k] != s_old) ? wave_sum(part[k]) : v_old;
+        fresh_v[k] = (k < z_h && nsel[k] != s_old) ? fresh_v[k] : v_old;
       __syncthreads();
       if (lane_h == 0) {
         occ[m] = s_new;
