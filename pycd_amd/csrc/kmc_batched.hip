@@ -669,10 +669,16 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     const float nikTf = float(-1.0 / td.kT);
     const float nuf = float(a.nu);
 
-    // uniforms of the next hop, drawn ahead while a hop's gathers are in
-    // flight; only committed (mti advanced) when that hop runs
-    bool pre_ok = false;
-    double pre_u1 = 0.0, pre_u2 = 0.0, pre_lu2 = 0.0;
+    // Uniforms of the coming hops, one hop per lane: at a refill (mti = m)
+    // lane l takes u1 and log(u2) (-inf for u2 = 0) of the hop that draws
+    // words m + 4 l .. m + 4 l + 3 of the MT block in LDS, and lane c_next
+    // holds the next hop's.  Only hops whose four words all lie in the
+    // block are cached, so the block is never regenerated ahead of the hops
+    // that consume it; a hop whose words straddle the block's end draws
+    // them directly.  Each lane takes one FP64 log per up to 64 hops
+    // instead of every lane taking one per hop.
+    double c_u1 = 0.0, c_lu2 = 0.0;
+    int c_next = 0, c_n = 0;
     for (;;) {
       // Conditions on the lane and on the trajectory's sizes are formed
       // again each hop from these copies, which the compiler cannot move
@@ -700,18 +706,28 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         __syncthreads();
       }
       // ---- two uniforms, as random.random() draws them
-      double u1, u2, lu2;
-      if (pre_ok) {
-        u1 = pre_u1;
-        u2 = pre_u2;
-        lu2 = pre_lu2;
+      if (c_next >= c_n && mti <= kMtN - 4) {
+        c_n = min(64, (kMtN - mti) / 4);
+        c_next = 0;
+        if (lane_h < c_n) {
+          const int w = mti + 4 * lane_h;
+          c_u1 = mt_uniform(mt_temper(mt[w]), mt_temper(mt[w + 1]));
+          const double u2 = mt_uniform(mt_temper(mt[w + 2]), mt_temper(mt[w + 3]));
+          c_lu2 = u2 > 0.0 ? log(u2) : -INFINITY;
+        }
+      }
+      double u1, lu2;
+      if (c_next < c_n) {
+        u1 = read_lane(c_u1, c_next);
+        lu2 = read_lane(c_lu2, c_next);
+        ++c_next;
         mti += 4;
       } else {
         uint32_t wd[4];
         mt_next4(mt, mti, lane_h, wd);
         u1 = mt_uniform(wd[0], wd[1]);
-        u2 = mt_uniform(wd[2], wd[3]);
-        lu2 = u2 > 0.0 ? log(u2) : 0.0;
+        const double u2 = mt_uniform(wd[2], wd[3]);
+        lu2 = u2 > 0.0 ? log(u2) : -INFINITY;
       }
       // ---- rates from shared memory only
       double kv[kPer], gv[kPer];
@@ -809,7 +825,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       const double ksel = read_lane(k_found, src);
       // ---- clock
       int64_t end;
-      if (u2 > 0.0) {
+      if (lu2 > -INFINITY) {  // u2 > 0
         t_clk -= lu2 / k_tot;
         const double qq = t_clk / td.dt;
         end = qq < 9.0e18 ? int64_t(qq) : td.P;
@@ -872,16 +888,6 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       }
       // hop vector of carrier m, one component per lane 0..2
       const double vec_own = lane_h < 3 ? ptab[rec_old].vec[lane_h] : 0.0;
-      // next hop's uniforms while the loads are in flight (not across a
-      // regeneration of the MT block)
-      pre_ok = mti <= kMtN - 4;
-      if (pre_ok) {
-        const uint32_t w0 = mt_temper(mt[mti]), w1 = mt_temper(mt[mti + 1]);
-        const uint32_t w2 = mt_temper(mt[mti + 2]), w3 = mt_temper(mt[mti + 3]);
-        pre_u1 = mt_uniform(w0, w1);
-        pre_u2 = mt_uniform(w2, w3);
-        pre_lu2 = pre_u2 > 0.0 ? log(pre_u2) : 0.0;
-      }
       // ---- apply (process m's old entries are dead once it is selected)
       if (lane_h < z_h) pinfo[m * z_h + lane_h] = info_new;
 #pragma unroll
