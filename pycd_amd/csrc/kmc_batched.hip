@@ -667,6 +667,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       next_refresh =
           hops == 0 ? a.refresh : (hops + a.refresh - 1) / a.refresh * a.refresh;
     const float nikTf = float(-1.0 / td.kT);
+    const double inv_dt = 1.0 / td.dt;
     const float nuf = float(a.nu);
 
     // Uniforms of the coming hops, one hop per lane: at a refill (mti = m)
@@ -827,7 +828,12 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       int64_t end;
       if (lu2 > -INFINITY) {  // u2 > 0
         t_clk -= lu2 / k_tot;
-        const double qq = t_clk / td.dt;
+        // path step int(t / dt) of the correctly rounded quotient: below
+        // 1e6, t (1/dt) is within 4e-10 of it, so more than 1e-9 away from
+        // an integer its integer part is the quotient's; otherwise divide
+        double qq = t_clk * inv_dt;
+        const double fr = qq - floor(qq);
+        if (!(fr > 1e-9 && fr < 1.0 - 1e-9 && qq < 1.0e6)) qq = t_clk / td.dt;
         end = qq < 9.0e18 ? int64_t(qq) : td.P;
       } else {
         t_clk = INFINITY;
@@ -845,18 +851,17 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       const uint32_t rec_old = info_sel >> 24;
       // table entries of the new site's processes and the slot
       // permutations of their targets (lanes < z), issued first
-      uint32_t info_new = 0;
+      uint32_t info_new = 0, nsite = 0;
       if (lane_h < z_h) {
         const uint32_t k = sub_new * z_h + ((perm_new >> (3 * lane_h)) & 7u);
-        info_new = (k << 24) |
-                   as_global(f.perm)[site_index(site_add(s_new, ptab[k].tgt, g), g)];
+        nsite = site_add(s_new, ptab[k].tgt, g);
+        info_new = (k << 24) | as_global(f.perm)[site_index(nsite, g)];
       }
       uint32_t nsel[kZp - 1];  // targets of the new site, in its slot order
 #pragma unroll
       for (int k = 0; k < kZp - 1; ++k)
-        nsel[k] = k < z_h ? uni(site_add(
-                              s_new, ptab[sub_new * z_h + ((perm_new >> (3 * k)) & 7u)].tgt, g))
-                        : 0u;
+        nsel[k] = k < z_h ? uint32_t(__builtin_amdgcn_readlane(int(nsite), k))
+                          : 0u;
       const gptr<TT> rn = crow<TT>(f, sub_new);
       const gptr<TT> ro = crow<TT>(f, s_old >> 24);
       const RowKey key_new = row_key(s_new, gp), key_old = row_key(s_old, gp);
