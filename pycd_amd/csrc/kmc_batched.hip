@@ -374,11 +374,19 @@ __device__ inline gptr<float> crow<float>(const FamDev& f, uint32_t sub) {
   return as_global(f.C32) + int64_t(sub) * f.N;
 }
 
+// row[col], addressed by a 32-bit byte offset (rows hold fewer than 2^24
+// entries) so that the load can take a scalar base and a vector offset.
+template <typename TT>
+__device__ inline TT row_at(gptr<TT> row, uint32_t col) {
+  using cbyte = const __attribute__((address_space(1))) char;
+  return *(gptr<TT>)((cbyte*)row + col * uint32_t(sizeof(TT)));
+}
+
 // A[a, b] for packed sites a, b.
 template <typename TT>
 __device__ inline double pair_entry(const FamDev& f, const GeoPk& p,
                                     uint32_t a, uint32_t b) {
-  return double(crow<TT>(f, a >> 24)[rel_col(a, b, p)]);
+  return double(row_at(crow<TT>(f, a >> 24), rel_col(a, b, p)));
 }
 
 // psi[r] = sum_c q A[sites[r], occ[c]] for every slot r (lanes over slots).
@@ -866,15 +874,18 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       const gptr<TT> ro = crow<TT>(f, s_old >> 24);
       const RowKey key_new = row_key(s_new, gp), key_old = row_key(s_old, gp);
       constexpr int kRPer = kMaxCarriers * kZp / 64;
-      double dv[kRPer];
+      // both entries of every slot are loaded before any is used, so that
+      // the slots' gathers are in flight together
+      TT an[kRPer], ao[kRPer];
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
         const int r = lane_h + 64 * i;
-        dv[i] = 0.0;
+        an[i] = TT(0);
+        ao[i] = TT(0);
         if (r < R_h) {
           const uint32_t x = sites[r];
-          dv[i] = double(rn[rel_col(key_new, x, gp)]) -
-                  double(ro[rel_col(key_old, x, gp)]);
+          an[i] = row_at(rn, rel_col(key_new, x, gp));
+          ao[i] = row_at(ro, rel_col(key_old, x, gp));
         }
       }
       // fresh sums of the new slots (not the one that returns to s_old):
@@ -898,7 +909,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
         const int r = lane_h + 64 * i;
-        if (r < R_h) psi[r] += q * dv[i];
+        if (r < R_h) psi[r] += q * (double(an[i]) - double(ao[i]));
       }
       if (lane_h < 3) {
         atomicAdd(sdisp + 3 * m + lane_h, vec_own);
