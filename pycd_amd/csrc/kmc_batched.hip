@@ -526,6 +526,17 @@ __global__ void __launch_bounds__(64) simd_probe(uint32_t* seen, int spin) {
   for (int i = 0; i < spin; ++i) __builtin_amdgcn_s_sleep(127);
 }
 
+// f(std::integral_constant<int, n>) for the run-time n in [1, kMax].
+template <int kMax, typename F>
+__device__ __forceinline__ void with_count(int n, F&& f) {
+  if constexpr (kMax >= 1) {
+    if (n == kMax)
+      f(std::integral_constant<int, kMax>{});
+    else
+      with_count<kMax - 1>(n, f);
+  }
+}
+
 // Wavefronts per SIMD the register allocation targets: 4 (16 per CU, every
 // trajectory of a 4096-trajectory batch resident at once, at the price of a
 // few spilled registers) for FP32 rates with at most 4 processes per lane;
@@ -745,10 +756,16 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       for (int i = 0; i < kPer; ++i) {
         kv[i] = 0.0;
         gv[i] = 0.0;
-        if (i < per_h) {
+      }
+      // one straight-line body per processes-per-lane count, so that the
+      // processes' LDS reads are in flight together
+      with_count<kPer>(per_h, [&](auto n_per) {
+        constexpr int kP = decltype(n_per)::value;
+#pragma unroll
+        for (int i = 0; i < kP; ++i) {
           const uint32_t d = pd[i];
           const bool real = d >> 31;
-          const int p = real ? lane_h * per_h + i : 0;
+          const int p = real ? lane_h * kP + i : 0;
           const PTab<kF32Rate>& e = ptab[pinfo[p] >> 24];
           const double dpsi =
               psi[(d >> 19) & 0x3ffu] - psi[(d >> 9) & 0x3ffu];
@@ -769,7 +786,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
           gv[i] = g0;
           ksum += kv[i];
         }
-      }
+      });
       // ---- FP64 prefix and selection.  FP64 rates: the prefix of the
       // quotients k / k_tot against u1, as the reference forms it; FP32
       // rates: one scan of the rates themselves against u1 k_tot.
@@ -890,16 +907,17 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       }
       // fresh sums of the new slots (not the one that returns to s_old):
       // one carrier per lane (n_c <= 64)
+      // (entries loaded here, used after the apply: the loads overlap)
       constexpr int kFresh = (kZp - 1 + 3) / 4 * 4;
-      double part[kFresh];
+      TT fent[kFresh];
       {
         const bool has = lane_h < nc_h;
         const uint32_t o = has ? (lane_h == m ? s_new : occ[lane_h]) : 0u;
 #pragma unroll
         for (int k = 0; k < kFresh; ++k) {
-          part[k] = 0.0;
+          fent[k] = TT(0);
           if (k < kZp - 1 && k < z_h && has && nsel[k] != s_old)
-            part[k] = q * pair_entry<TT>(f, gp, nsel[k], o);
+            fent[k] = row_at(crow<TT>(f, nsel[k] >> 24), rel_col(nsel[k], o, gp));
         }
       }
       // hop vector of carrier m, one component per lane 0..2
@@ -918,7 +936,9 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       __syncthreads();
       const double v_new = uni(psi[m * zp + 1 + j]);
       const double v_old = uni(psi[m * zp]);
-      double fresh_v[kFresh];
+      double part[kFresh], fresh_v[kFresh];
+#pragma unroll
+      for (int k = 0; k < kFresh; ++k) part[k] = q * double(fent[k]);
 #pragma unroll
       for (int k = 0; k < kFresh; k += 4) wave_sum4(part + k, fresh_v + k);
 #pragma unroll
