@@ -1019,12 +1019,14 @@ kmc_pool_kernel(BatchArgs a) {
 // Squared displacement per trajectory, averaged over its carriers:
 //   msd[t, tau] = mean_c mean_{t0 < P_t - tau} |r_c(t0 + tau) - r_c(t0)|^2
 // One workgroup of 256 per (trajectory, block of kMsdBlock lags).  Each
-// 16-lane row of a wavefront takes one group of 8 lags at a time, its lanes
-// 16 consecutive carriers (so a row's loads are contiguous in the path
-// rows), and walks t0 in chunks of 4 with the 12 positions
-// r(t0 + tau0 .. t0 + tau0 + 11) of the chunk in registers: a chunk loads 8
-// positions and forms 32 squared distances.  The row sums its carriers with
-// DPP and its last lane writes the 8 lags.
+// 16-lane row of a wavefront takes one group of kMsdLags lags at a time,
+// its lanes 16 consecutive carriers (so a row's loads are contiguous in the
+// path rows), and walks t0 in chunks of kMsdChunk with the positions
+// r(t0 + tau0 .. t0 + tau0 + kMsdLags + kMsdChunk - 1) of the chunk in
+// registers: a chunk loads 2 kMsdChunk positions and forms
+// kMsdLags kMsdChunk squared distances.  Distances are summed unscaled and
+// scaled by scale^2 once.  The row sums its carriers with DPP and its last
+// lane writes the group's lags.
 struct MsdTraj {
   int64_t row_off;
   int64_t P;
@@ -1034,8 +1036,9 @@ struct MsdTraj {
   int64_t M;
 };
 
-constexpr int kMsdBlock = 512;  // lags per workgroup (64 groups of 8)
-constexpr int kMsdChunk = 4;    // t0 per chunk
+constexpr int kMsdLags = 16;    // lags per row
+constexpr int kMsdChunk = 2;    // t0 per chunk
+constexpr int kMsdBlock = 512;  // lags per workgroup (2 passes of 16 rows)
 
 template <bool kMasked>
 __device__ inline void msd_chunk(const double* wx, const double* wy,
@@ -1046,15 +1049,11 @@ __device__ inline void msd_chunk(const double* wx, const double* wy,
 #pragma unroll
   for (int s = 0; s < kMsdChunk; ++s)
 #pragma unroll
-    for (int l = 0; l < 8; ++l) {
+    for (int l = 0; l < kMsdLags; ++l) {
       const double dx = wx[s + l] - x0[s], dy = wy[s + l] - y0[s],
                    dz = wz[s + l] - z0[s];
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      if (kMasked) {
-        if (((lmask >> l) & 1u) && T + s + l < tmax) acc[l] += d2;
-      } else {
-        acc[l] += d2;
-      }
+      if (!kMasked || (((lmask >> l) & 1u) && T + s + l < tmax))
+        acc[l] = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[l])));
     }
 }
 
@@ -1073,6 +1072,8 @@ msd_batched_kernel(const double* __restrict__ rows,
                    const MsdTraj* __restrict__ tr,
                    const TrajOut* __restrict__ outs, int n_traj, int nlb_max,
                    double scale, double* __restrict__ out) {
+  constexpr int L = kMsdLags, C = kMsdChunk;
+  constexpr int kGroups = kMsdBlock / L;  // lag groups per workgroup
   const int t = blockIdx.x / nlb_max;
   const int lb = blockIdx.x - t * nlb_max;
   if (t >= n_traj) return;
@@ -1085,46 +1086,46 @@ msd_batched_kernel(const double* __restrict__ rows,
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = lane & 15, rw = lane >> 4;
   const int w = 3 * d.nc;
-  constexpr int C = kMsdChunk;
-  // 64 groups of 8 lags: 16 rows (4 per wavefront) x 4 passes
-  for (int pass = 0; pass < 4; ++pass) {
+  const double s2 = scale * scale;
+  // kGroups groups of L lags: 16 rows (4 per wavefront) per pass
+  for (int pass = 0; pass < kGroups / 16; ++pass) {
     const int grp = pass * 16 + wave * 4 + rw;
-    const int64_t tau0 = blk0 + 8 * grp;
-    if (blk0 + 8 * (pass * 16 + wave * 4) >= d.M) break;  // whole wavefront
+    const int64_t tau0 = blk0 + int64_t(L) * grp;
+    if (blk0 + int64_t(L) * (pass * 16 + wave * 4) >= d.M) break;  // wavefront
     uint32_t lmask = 0;  // lags of this row inside [1, M)
 #pragma unroll
-    for (int l = 0; l < 8; ++l)
+    for (int l = 0; l < L; ++l)
       if (tau0 + l >= 1 && tau0 + l < d.M) lmask |= 1u << l;
     const int64_t tmax = P - tau0;  // t0 < tmax for lag tau0
-    double acc[8];
+    double acc[L];
 #pragma unroll
-    for (int l = 0; l < 8; ++l) acc[l] = 0.0;
+    for (int l = 0; l < L; ++l) acc[l] = 0.0;
     if (lmask && tmax > 0) {
       for (int c = cl; c < d.nc; c += 16) {
         const double* pc = rows + d.row_off + 3 * c;
         // position at path step tt (clamped into the written rows)
         auto load = [&](int64_t tt, double& x, double& y, double& z) {
           const double* q = pc + (tt < P ? tt : P - 1) * w;
-          x = q[0] * scale;
-          y = q[1] * scale;
-          z = q[2] * scale;
+          x = q[0];
+          y = q[1];
+          z = q[2];
         };
-        double wx[8 + C], wy[8 + C], wz[8 + C];
+        double wx[L + C], wy[L + C], wz[L + C];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) load(tau0 + k, wx[k], wy[k], wz[k]);
+        for (int k = 0; k < L; ++k) load(tau0 + k, wx[k], wy[k], wz[k]);
         for (int64_t T = 0; T < tmax; T += C) {
           double x0[C], y0[C], z0[C];
 #pragma unroll
           for (int k = 0; k < C; ++k) {
-            load(tau0 + T + 8 + k, wx[8 + k], wy[8 + k], wz[8 + k]);
+            load(tau0 + T + L + k, wx[L + k], wy[L + k], wz[L + k]);
             load(T + k, x0[k], y0[k], z0[k]);
           }
-          if (lmask == 0xffu && T + (C - 1) + 7 < tmax)
+          if (lmask == (1u << L) - 1u && T + (C - 1) + (L - 1) < tmax)
             msd_chunk<false>(wx, wy, wz, x0, y0, z0, T, tmax, lmask, acc);
           else
             msd_chunk<true>(wx, wy, wz, x0, y0, z0, T, tmax, lmask, acc);
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
+          for (int k = 0; k < L; ++k) {
             wx[k] = wx[k + C];
             wy[k] = wy[k + C];
             wz[k] = wz[k + C];
@@ -1134,14 +1135,15 @@ msd_batched_kernel(const double* __restrict__ rows,
     }
     // every lane of the wavefront takes part in the row sums
 #pragma unroll
-    for (int l = 0; l < 8; ++l) acc[l] = row_sum16(acc[l]);
+    for (int l = 0; l < L; ++l) acc[l] = row_sum16(acc[l]);
     if (cl == 15) {
 #pragma unroll
-      for (int l = 0; l < 8; ++l) {
+      for (int l = 0; l < L; ++l) {
         const int64_t tau = tau0 + l;
         if (tau >= d.M) break;
         double v = 0.0;
-        if (tau >= 1 && tau < P) v = acc[l] / double(P - tau) / double(d.nc);
+        if (tau >= 1 && tau < P)
+          v = acc[l] * s2 / double(P - tau) / double(d.nc);
         out[d.out_off + tau] = v;
       }
     }
