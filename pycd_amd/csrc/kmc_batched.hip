@@ -1101,29 +1101,68 @@ msd_batched_kernel(const double* __restrict__ rows,
 #pragma unroll
     for (int l = 0; l < L; ++l) acc[l] = 0.0;
     if (lmask && tmax > 0) {
+      // path rows of this trajectory: a scalar base and 32-bit per-lane
+      // byte offsets (a trajectory's rows span less than 4 GiB)
+      const gptr<char> row0 = (gptr<char>)(rows + d.row_off);
+      const uint32_t stride = uint32_t(w) * 8u;
+      const int64_t last = P - 1;
       for (int c = cl; c < d.nc; c += 16) {
-        const double* pc = rows + d.row_off + 3 * c;
-        // position at path step tt (clamped into the written rows)
-        auto load = [&](int64_t tt, double& x, double& y, double& z) {
-          const double* q = pc + (tt < P ? tt : P - 1) * w;
+        const uint32_t cb = uint32_t(c) * 24u;
+        auto load = [&](uint32_t off, double& x, double& y, double& z) {
+          const gptr<double> q = (gptr<double>)(row0 + off);
           x = q[0];
           y = q[1];
           z = q[2];
         };
+        // position at path step tt, clamped into the written rows
+        auto load_at = [&](int64_t tt, double& x, double& y, double& z) {
+          load(cb + uint32_t(tt < last ? tt : last) * stride, x, y, z);
+        };
         double wx[L + C], wy[L + C], wz[L + C];
 #pragma unroll
-        for (int k = 0; k < L; ++k) load(tau0 + k, wx[k], wy[k], wz[k]);
-        for (int64_t T = 0; T < tmax; T += C) {
+        for (int k = 0; k < L; ++k) load_at(tau0 + k, wx[k], wy[k], wz[k]);
+        int64_t T = 0;
+        // Whole turns of the window ring: kU chunks advance it by its
+        // length, so slot (j C + i) % kW of chunk j holds r(tau0 + T + j C
+        // + i) and no position is moved between registers; every load and
+        // every distance of a turn is in range.
+        constexpr int kW = L + C, kU = kW / C;
+        static_assert(kW % C == 0, "the ring turns in whole chunks");
+        if (lmask == (1u << L) - 1u) {
+          uint32_t ow = cb + uint32_t(tau0 + L) * stride, oo = cb;
+          for (; T + kU * C + L - 1 < tmax; T += kU * C) {
+#pragma unroll
+            for (int j = 0; j < kU; ++j) {
+              double x0[C], y0[C], z0[C];
+#pragma unroll
+              for (int k = 0; k < C; ++k) {
+                const int sl = (j * C + L + k) % kW;
+                load(ow, wx[sl], wy[sl], wz[sl]);
+                load(oo, x0[k], y0[k], z0[k]);
+                ow += stride;
+                oo += stride;
+              }
+#pragma unroll
+              for (int s2i = 0; s2i < C; ++s2i)
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                  const int sl = (j * C + s2i + l) % kW;
+                  const double dx = wx[sl] - x0[s2i], dy = wy[sl] - y0[s2i],
+                               dz = wz[sl] - z0[s2i];
+                  acc[l] = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[l])));
+                }
+            }
+          }
+        }
+        // the rest chunk by chunk, the window shifted through registers
+        for (; T < tmax; T += C) {
           double x0[C], y0[C], z0[C];
 #pragma unroll
           for (int k = 0; k < C; ++k) {
-            load(tau0 + T + L + k, wx[L + k], wy[L + k], wz[L + k]);
-            load(T + k, x0[k], y0[k], z0[k]);
+            load_at(tau0 + T + L + k, wx[L + k], wy[L + k], wz[L + k]);
+            load_at(T + k, x0[k], y0[k], z0[k]);
           }
-          if (lmask == (1u << L) - 1u && T + (C - 1) + (L - 1) < tmax)
-            msd_chunk<false>(wx, wy, wz, x0, y0, z0, T, tmax, lmask, acc);
-          else
-            msd_chunk<true>(wx, wy, wz, x0, y0, z0, T, tmax, lmask, acc);
+          msd_chunk<true>(wx, wy, wz, x0, y0, z0, T, tmax, lmask, acc);
 #pragma unroll
           for (int k = 0; k < L; ++k) {
             wx[k] = wx[k + C];
