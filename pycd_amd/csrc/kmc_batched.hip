@@ -1106,7 +1106,14 @@ msd_batched_kernel(const double* __restrict__ rows,
       const gptr<char> row0 = (gptr<char>)(rows + d.row_off);
       const uint32_t stride = uint32_t(w) * 8u;
       const int64_t last = P - 1;
-      for (int c = cl; c < d.nc; c += 16) {
+      for (int c0 = 0; c0 < d.nc; c0 += 16) {
+        // a block of nb < 16 carriers is repeated over ph = 16 / nb groups
+        // of lanes, each taking a contiguous share of the origins
+        const int nb = min(16, d.nc - c0), ph = 16 / nb;
+        if (cl >= nb * ph) continue;
+        const int c = c0 + cl % nb, phase = cl / nb;
+        const int64_t seg = ((tmax + ph - 1) / ph + C - 1) / C * C;
+        const int64_t T0 = min(tmax, phase * seg), T1 = min(tmax, T0 + seg);
         const uint32_t cb = uint32_t(c) * 24u;
         auto load = [&](uint32_t off, double& x, double& y, double& z) {
           const gptr<double> q = (gptr<double>)(row0 + off);
@@ -1120,8 +1127,9 @@ msd_batched_kernel(const double* __restrict__ rows,
         };
         double wx[L + C], wy[L + C], wz[L + C];
 #pragma unroll
-        for (int k = 0; k < L; ++k) load_at(tau0 + k, wx[k], wy[k], wz[k]);
-        int64_t T = 0;
+        for (int k = 0; k < L; ++k)
+          load_at(tau0 + T0 + k, wx[k], wy[k], wz[k]);
+        int64_t T = T0;
         // Whole turns of the window ring: kU chunks advance it by its
         // length, so slot (j C + i) % kW of chunk j holds r(tau0 + T + j C
         // + i) and no position is moved between registers; every load and
@@ -1129,8 +1137,9 @@ msd_batched_kernel(const double* __restrict__ rows,
         constexpr int kW = L + C, kU = kW / C;
         static_assert(kW % C == 0, "the ring turns in whole chunks");
         if (lmask == (1u << L) - 1u) {
-          uint32_t ow = cb + uint32_t(tau0 + L) * stride, oo = cb;
-          for (; T + kU * C + L - 1 < tmax; T += kU * C) {
+          uint32_t ow = cb + uint32_t(tau0 + T0 + L) * stride,
+                   oo = cb + uint32_t(T0) * stride;
+          for (; T + kU * C + L - 1 < tmax && T + kU * C <= T1; T += kU * C) {
 #pragma unroll
             for (int j = 0; j < kU; ++j) {
               double x0[C], y0[C], z0[C];
@@ -1155,7 +1164,7 @@ msd_batched_kernel(const double* __restrict__ rows,
           }
         }
         // the rest chunk by chunk, the window shifted through registers
-        for (; T < tmax; T += C) {
+        for (; T < T1; T += C) {
           double x0[C], y0[C], z0[C];
 #pragma unroll
           for (int k = 0; k < C; ++k) {
