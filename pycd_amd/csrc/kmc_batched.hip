@@ -114,6 +114,9 @@ struct BatchArgs {
   const int16_t* key_bin;
   int32_t n_bins;
   int32_t pad1;
+  // remaining hops of the wavefront in each SIMD wave slot (kSimdKeys x 16,
+  // zeroed before the launch), for pacing; NULL: no pacing
+  uint32_t* progress;
   // resumable state (slices): MT words, occupancy, displacement/base/drift
   uint32_t* mts;            // n_traj x 625
   uint32_t* st_occ;         // n_traj x kMaxCarriers (packed sites)
@@ -490,6 +493,39 @@ __device__ inline uint32_t simd_key() {
 }
 constexpr int kSimdKeys = 1 << 13;
 
+// simd_key() << 4 | the wavefront's slot in its SIMD (HW_ID bits 3:0).
+__device__ inline uint32_t wave_slot() {
+  return (simd_key() << 4) | (__builtin_amdgcn_s_getreg((3 << 11) | 4) & 15u);
+}
+
+// Wavefronts sharing a SIMD pace each other.  Each publishes the hops it
+// has left and takes a priority by rank among its SIMD's wavefronts, most
+// hops left highest, so that the SIMD's trajectories advance together and
+// end together; left alone, the SIMD favours its older wavefronts, whose
+// trajectories end first while the rest finish on a half-idle SIMD.
+__device__ inline void pace(uint32_t* prog, uint32_t slot, uint32_t left,
+                            int lane) {
+  if (lane == 0)
+    __hip_atomic_store(prog + slot, left, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t base = slot & ~15u, me = slot & 15u;
+  uint32_t v = 0;
+  if (lane < 16)
+    v = __hip_atomic_load(prog + base + lane, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+  const bool before = lane < 16 && uint32_t(lane) != me &&
+                      (v > left || (v == left && uint32_t(lane) < me));
+  const int rank = __popcll(__ballot(before));
+  if (rank == 0)
+    __builtin_amdgcn_s_setprio(3);
+  else if (rank == 1)
+    __builtin_amdgcn_s_setprio(2);
+  else if (rank == 2)
+    __builtin_amdgcn_s_setprio(1);
+  else
+    __builtin_amdgcn_s_setprio(0);
+}
+
 // Index into `order` of this wavefront's next trajectory, or n_traj once
 // the queue is empty (lane 0 only).  Pops from `bin` first, then moves on
 // to the next bins; stops scanning as soon as every trajectory is taken.
@@ -558,6 +594,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
   uint32_t* cperm = occ + a.nc_max;
   uint32_t* pinfo = cperm + a.nc_max;
 
+  const uint32_t pslot = a.progress ? wave_slot() : 0u;
   int bin = 0;
   if (a.n_bins > 1) {
     const int b = a.key_bin[simd_key()];
@@ -699,6 +736,11 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     // instead of every lane taking one per hop.
     double c_u1 = 0.0, c_lu2 = 0.0;
     int c_next = 0, c_n = 0;
+    // hops left to run, as pacing counts them
+    auto hops_left = [&]() {
+      const int64_t h = td.max_hops > 0 ? td.max_hops - hops : INT32_MAX;
+      return uint32_t(h < INT32_MAX ? h : INT32_MAX);
+    };
     for (;;) {
       // Conditions on the lane and on the trajectory's sizes are formed
       // again each hop from these copies, which the compiler cannot move
@@ -719,6 +761,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       }
       if (a.slice_hops > 0 && slice_done >= a.slice_hops) break;
       ++slice_done;
+      if (a.progress && (hops & 63) == 0) pace(a.progress, pslot, hops_left(), lane);
       if (hops == next_refresh) {
         next_refresh += a.refresh;
         // bound the rounding carried by the incremental updates
@@ -975,6 +1018,12 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         finished = true;
         break;
       }
+    }
+    if (a.progress) {  // nothing left: out of its SIMD mates' way
+      if (lane == 0)
+        __hip_atomic_store(a.progress + pslot, 0u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_setprio(0);
     }
     // ---- outputs and resumable state
     for (int k = lane; k < w; k += 64)
@@ -1282,6 +1331,7 @@ struct pycd_batch {
   DevBuf<int64_t> d_occ;
   DevBuf<uint32_t> d_mt, d_counter, d_bins;
   DevBuf<int16_t> d_keybin;
+  DevBuf<uint32_t> d_prog;  // pacing slots (batch runs with several bins)
   int n_bins = 1;
   DevBuf<double> d_rows, d_drift, d_msd;
   DevBuf<TrajOut> d_out;
