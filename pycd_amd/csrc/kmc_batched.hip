@@ -736,9 +736,10 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     // instead of every lane taking one per hop.
     double c_u1 = 0.0, c_lu2 = 0.0;
     int c_next = 0, c_n = 0;
-    // hops left to run, as pacing counts them
+    // hops left to run in this launch, as pacing counts them
     auto hops_left = [&]() {
-      const int64_t h = td.max_hops > 0 ? td.max_hops - hops : INT32_MAX;
+      int64_t h = td.max_hops > 0 ? td.max_hops - hops : INT32_MAX;
+      if (a.slice_hops > 0) h = min(h, a.slice_hops - slice_done);
       return uint32_t(h < INT32_MAX ? h : INT32_MAX);
     };
     for (;;) {
@@ -1331,7 +1332,7 @@ struct pycd_batch {
   DevBuf<int64_t> d_occ;
   DevBuf<uint32_t> d_mt, d_counter, d_bins;
   DevBuf<int16_t> d_keybin;
-  DevBuf<uint32_t> d_prog;  // pacing slots (batch runs with several bins)
+  DevBuf<uint32_t> d_prog;  // pacing slots, one per SIMD wave slot
   int n_bins = 1;
   DevBuf<double> d_rows, d_drift, d_msd;
   DevBuf<TrajOut> d_out;
