@@ -1332,7 +1332,7 @@ struct pycd_batch {
   DevBuf<int64_t> d_occ;
   DevBuf<uint32_t> d_mt, d_counter, d_bins;
   DevBuf<int16_t> d_keybin;
-  DevBuf<uint32_t> d_prog;  // pacing slots, one per SIMD wave slot
+  DevBuf<uint32_t> d_prog;  // pacing slots, one per SIMD wave slot (binned runs)
   int n_bins = 1;
   DevBuf<double> d_rows, d_drift, d_msd;
   DevBuf<TrajOut> d_out;
