@@ -215,7 +215,8 @@ int pycd_msd(pycd_ctx* ctx, const double* pos, int64_t n_traj,
  * n_unit + sub (Material.generate_sites, PyCD/core.py:154-241): the library
  * checks that every per-site value, hop slot and (with table_rows = N)
  * table entry is the cell-0 one moved by the site's cell, and keeps only
- * the table rows of cell 0's sites on the device (n_unit x N). */
+ * the table rows of cell 0's sites on the device (n_unit x N).  N must be
+ * below 2^24 and cells[1] cells[2] n_unit below 65536. */
 typedef struct {
   int64_t n_sites;            /* N = cells[0] cells[1] cells[2] n_unit      */
   int32_t z;                  /* hop slots per hopping site (1..7)          */
