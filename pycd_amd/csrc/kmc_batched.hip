@@ -377,24 +377,36 @@ __device__ inline gptr<float> crow<float>(const FamDev& f, uint32_t sub) {
   return as_global(f.C32) + int64_t(sub) * f.N;
 }
 
-// row[col], addressed by a 32-bit byte offset (rows hold fewer than 2^24
-// entries) so that the load can take a scalar base and a vector offset.
+// The family's compact table (n_unit rows of N entries) reached by 32-bit
+// byte offsets from one base, so that every gather takes the scalar base
+// and one vector offset (the family check keeps the table under 4 GiB).
 template <typename TT>
-__device__ inline TT row_at(gptr<TT> row, uint32_t col) {
-  using cbyte = const __attribute__((address_space(1))) char;
-  return *(gptr<TT>)((cbyte*)row + col * uint32_t(sizeof(TT)));
+struct Tab {
+  gptr<char> base;
+  uint32_t row_bytes;  // N sizeof(TT)
+  __device__ uint32_t row(uint32_t sub) const { return sub * row_bytes; }
+  __device__ TT at(uint32_t row_off, uint32_t col) const {
+    return *(gptr<TT>)(base + (row_off + col * uint32_t(sizeof(TT))));
+  }
+};
+template <typename TT>
+__device__ inline Tab<TT> tab_of(const FamDev& f) {
+  Tab<TT> t;
+  t.base = (gptr<char>)crow<TT>(f, 0);
+  t.row_bytes = uint32_t(f.N) * uint32_t(sizeof(TT));
+  return t;
 }
 
 // A[a, b] for packed sites a, b.
 template <typename TT>
-__device__ inline double pair_entry(const FamDev& f, const GeoPk& p,
+__device__ inline double pair_entry(const Tab<TT>& t, const GeoPk& p,
                                     uint32_t a, uint32_t b) {
-  return double(row_at(crow<TT>(f, a >> 24), rel_col(a, b, p)));
+  return double(t.at(t.row(a >> 24), rel_col(a, b, p)));
 }
 
 // psi[r] = sum_c q A[sites[r], occ[c]] for every slot r (lanes over slots).
 template <typename TT>
-__device__ inline void psi_from_table(const FamDev& f, const GeoPk& gp,
+__device__ inline void psi_from_table(const Tab<TT>& t, const GeoPk& gp,
                                       double q, int R, int nc,
                                       const uint32_t* sites,
                                       const uint32_t* occ, double* psi,
@@ -402,11 +414,11 @@ __device__ inline void psi_from_table(const FamDev& f, const GeoPk& gp,
   for (int r = lane; r < R; r += 64) {
     const uint32_t a = sites[r];
     const RowKey key = row_key(a, gp);
-    const gptr<TT> row = crow<TT>(f, key.sub);
+    const uint32_t row = t.row(key.sub);
     double acc = 0.0;
 #pragma unroll 4
     for (int c2 = 0; c2 < nc; ++c2)
-      acc = fma(q, double(row[rel_col(key, occ[c2], gp)]), acc);
+      acc = fma(q, double(t.at(row, rel_col(key, occ[c2], gp))), acc);
     psi[r] = acc;
   }
 }
@@ -615,6 +627,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     const FamDev f = a.fams[uni(td.fam)];
     const Geo g{uni(f.nx), uni(f.ny), uni(f.nz), uni(f.nu)};
     const GeoPk gp = geo_pk(g);
+    const Tab<TT> tab = tab_of<TT>(f);
     const int nc = uni(td.nc), z = uni(f.z), zp = z + 1;
     const int R = nc * zp, nproc = nc * z;
     const int per = (nproc + 63) / 64;
@@ -698,14 +711,14 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       hops = o0.hops;
       no_select = o0.no_select;
     } else {
-      psi_from_table<TT>(f, gp, q, R, nc, sites, occ, psi, lane);
+      psi_from_table<TT>(tab, gp, q, R, nc, sites, occ, psi, lane);
       // initial energy: ewald_neut + q_L A q_L + sum_c q (2 phi_L + psi_self)
       double e_loc = 0.0;
       for (int c = lane; c < nc; c += 64) {
         const uint32_t s = occ[c];
         double inner = 0.0;
         for (int c2 = 0; c2 < nc; ++c2)
-          inner = fma(q, pair_entry<TT>(f, gp, s, occ[c2]), inner);
+          inner = fma(q, pair_entry<TT>(tab, gp, s, occ[c2]), inner);
         e_loc += q * (2.0 * as_global(f.phiL)[s >> 24] + inner);
       }
       energy = td.ewald_neut + (f.qLAqL + wave_sum(e_loc));
@@ -766,7 +779,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       if (hops == next_refresh) {
         next_refresh += a.refresh;
         // bound the rounding carried by the incremental updates
-        psi_from_table<TT>(f, gp, q, R_h, nc_h, sites, occ, psi, lane_h);
+        psi_from_table<TT>(tab, gp, q, R_h, nc_h, sites, occ, psi, lane_h);
         __syncthreads();
       }
       // ---- two uniforms, as random.random() draws them
@@ -931,8 +944,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       for (int k = 0; k < kZp - 1; ++k)
         nsel[k] = k < z_h ? uint32_t(__builtin_amdgcn_readlane(int(nsite), k))
                           : 0u;
-      const gptr<TT> rn = crow<TT>(f, sub_new);
-      const gptr<TT> ro = crow<TT>(f, s_old >> 24);
+      const uint32_t rn = tab.row(sub_new), ro = tab.row(s_old >> 24);
       const RowKey key_new = row_key(s_new, gp), key_old = row_key(s_old, gp);
       constexpr int kRPer = kMaxCarriers * kZp / 64;
       // both entries of every slot are loaded before any is used, so that
@@ -945,8 +957,8 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         ao[i] = TT(0);
         if (r < R_h) {
           const uint32_t x = sites[r];
-          an[i] = row_at(rn, rel_col(key_new, x, gp));
-          ao[i] = row_at(ro, rel_col(key_old, x, gp));
+          an[i] = tab.at(rn, rel_col(key_new, x, gp));
+          ao[i] = tab.at(ro, rel_col(key_old, x, gp));
         }
       }
       // fresh sums of the new slots (not the one that returns to s_old):
@@ -961,7 +973,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         for (int k = 0; k < kFresh; ++k) {
           fent[k] = TT(0);
           if (k < kZp - 1 && k < z_h && has && nsel[k] != s_old)
-            fent[k] = row_at(crow<TT>(f, nsel[k] >> 24), rel_col(nsel[k], o, gp));
+            fent[k] = tab.at(tab.row(nsel[k] >> 24), rel_col(nsel[k], o, gp));
         }
       }
       // hop vector of carrier m, one component per lane 0..2
