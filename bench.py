@@ -180,8 +180,20 @@ def run_plain(args, log):
     hops = sess.read()["hops"]
     sess.close()
     dev.free()
+    # algorithmic bytes per hop of the plain kernel: the n_c (1 + z) x n_c
+    # table gathers of psi, 11 doubles per process (slot tables, phi_L, E,
+    # hop vector) and the two uniforms
+    nc = 16
+    z = int(np.max(run.nbr_count))
+    per_hop = 8 * nc * (1 + z) * nc + 8 * 11 * nc * z + 16
+    gbs = per_hop * hops / (ms / 1e3) / 1e9
     return {"hops": hops, "ms": ms, "hops_per_s": hops / (ms / 1e3),
-            "n_sites": w.n_sites, "n_carriers": 16}
+            "n_sites": w.n_sites, "n_carriers": nc, "z": z,
+            "roofline": {"bound": "hbm", "achieved": gbs,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS,
+                         "kernel": "kmc_plain_kernel",
+                         "algorithmic_bytes_per_hop": per_hop}}
 
 
 def run_serve(args, log):
@@ -415,6 +427,7 @@ def main():
             "waves_per_cu": info["waves_per_cu"],
         },
         "plain_hops_per_s": plain["hops_per_s"] if plain else None,
+        "plain_roofline": plain["roofline"] if plain else None,
         "batched_over_plain": (info["hops_per_s"] / plain["hops_per_s"]
                                if plain else None),
         "p50_latency_ms": serve["p50_latency_ms"] if serve else None,

@@ -775,7 +775,10 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       }
       if (a.slice_hops > 0 && slice_done >= a.slice_hops) break;
       ++slice_done;
-      if (a.progress && (hops & 63) == 0) pace(a.progress, pslot, hops_left(), lane);
+      if (a.progress && (hops & 63) == 0) {
+        asm volatile("");  // keeps hops_left() from being computed every hop
+        pace(a.progress, pslot, hops_left(), lane);
+      }
       if (hops == next_refresh) {
         next_refresh += a.refresh;
         // bound the rounding carried by the incremental updates
@@ -913,10 +916,15 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         // path step int(t / dt) of the correctly rounded quotient: below
         // 1e6, t (1/dt) is within 4e-10 of it, so more than 1e-9 away from
         // an integer its integer part is the quotient's; otherwise divide
-        double qq = t_clk * inv_dt;
-        const double fr = qq - floor(qq);
-        if (!(fr > 1e-9 && fr < 1.0 - 1e-9 && qq < 1.0e6)) qq = t_clk / td.dt;
-        end = qq < 9.0e18 ? int64_t(qq) : td.P;
+        const double qf = t_clk * inv_dt;
+        const double fr = qf - floor(qf);
+        if (fr > 1e-9 && fr < 1.0 - 1e-9 && qf < 1.0e6) {
+          end = int32_t(qf);
+        } else {
+          asm volatile("");  // the division only where it is needed
+          const double qq = t_clk / td.dt;
+          end = qq < 9.0e18 ? int64_t(qq) : td.P;
+        }
       } else {
         t_clk = INFINITY;
         end = td.P;
