@@ -21,6 +21,7 @@
 //      selection, clock, state update                 (wavefront 0)
 //   D  path-step rows written for [start, end)        (all threads)
 #include "common.hpp"
+#include "wave.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -65,17 +66,21 @@ struct KmcDev {
 constexpr int kThreads = 256;
 constexpr double kE = 2.718281828459045;  // numpy's e
 
-__device__ inline double wave_incl_scan(double v, int lane) {
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double o = __shfl_up(v, off, 64);
-    if (lane >= off) v += o;
+// log(u2) of every uniform pair ahead of the hop loop (0 where u2 = 0, a
+// value the loop does not use), so that no hop waits for an FP64 log.
+__global__ void __launch_bounds__(256)
+log_u2_kernel(const double* __restrict__ U, int64_t n,
+              double* __restrict__ lu) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double u2 = U[2 * i + 1];
+    lu[i] = u2 > 0.0 ? log(u2) : 0.0;
   }
-  return v;
 }
 
 __global__ void __launch_bounds__(kThreads)
-kmc_plain_kernel(KmcDev d, const double* __restrict__ U, int64_t n_pairs,
+kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
+                 const double* __restrict__ LU, int64_t n_pairs,
                  int64_t max_hops, double* __restrict__ hop_time,
                  int32_t* __restrict__ hop_proc, int32_t* __restrict__ status) {
   extern __shared__ double smem[];
@@ -173,14 +178,12 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U, int64_t n_pairs,
       const int p1 = min(p0 + per, nproc);
       double ksum = 0.0;
       for (int p = p0; p < p1; ++p) ksum += kk[p];
-      const double k_tot = __shfl(wave_incl_scan(ksum, lane), 63, 64);
+      const double k_tot = read_lane(wave_incl_scan(ksum), 63);
       const double u1 = U[2 * h];
       const double u2 = U[2 * h + 1];
       double xsum = 0.0;
       for (int p = p0; p < p1; ++p) xsum += kk[p] / k_tot;
-      const double xincl = wave_incl_scan(xsum, lane);
-      double cum = __shfl_up(xincl, 1, 64);
-      if (lane == 0) cum = 0.0;
+      double cum = wave_excl_scan(xsum);
       int found = -1;
       for (int p = p0; p < p1; ++p) {
         cum += kk[p] / k_tot;
@@ -207,7 +210,7 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U, int64_t n_pairs,
         double t = sh_t;
         int64_t end;
         if (u2 > 0.0) {
-          t -= log(u2) / k_tot;
+          t -= LU[h] / k_tot;
           const double q = t / d.dt;
           end = q < 9.0e18 ? int64_t(q) : d.P;
         } else {
@@ -372,7 +375,7 @@ struct pycd_kmc {
   DevBuf<int32_t> nbr, occ, status;
   DevBuf<double> disp, base, drift, rows, epath, dpath, tmp;
   DevBuf<KmcScalars> sc;
-  DevBuf<double> ubuf, tbuf;
+  DevBuf<double> ubuf, tbuf, lubuf;
   DevBuf<int32_t> pbuf;
   double qLAqL = 0.0;
   size_t smem = 0;
@@ -649,9 +652,15 @@ extern "C" int pycd_kmc_advance(pycd_kmc* s, const double* uniforms,
     PYCD_HIP(ctx, hipEventCreate(&s->ev0));
     PYCD_HIP(ctx, hipEventCreate(&s->ev1));
   }
+  if (s->lubuf.count < size_t(std::max<int64_t>(n_pairs, 1)))
+    PYCD_HIP(ctx, s->lubuf.alloc(size_t(std::max<int64_t>(n_pairs, 1))));
   PYCD_HIP(ctx, hipEventRecord(s->ev0, st));
-  kmc_plain_kernel<<<1, kThreads, s->smem, st>>>(d, dU, n_pairs, max_hops, dT,
-                                                 dP, s->status.ptr);
+  if (n_pairs > 0)
+    log_u2_kernel<<<int(std::min<int64_t>((n_pairs + 255) / 256, 4096)), 256,
+                    0, st>>>(dU, n_pairs, s->lubuf.ptr);
+  kmc_plain_kernel<<<1, kThreads, s->smem, st>>>(d, dU, s->lubuf.ptr, n_pairs,
+                                                 max_hops, dT, dP,
+                                                 s->status.ptr);
   PYCD_HIP(ctx, hipGetLastError());
   PYCD_HIP(ctx, hipEventRecord(s->ev1, st));
   int32_t status[2] = {-1, 0};
