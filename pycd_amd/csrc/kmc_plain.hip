@@ -91,14 +91,45 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
   double* disp = dg + nproc;          // 3 nc
   double* base = disp + 3 * nc;       // 3 nc
   double* drift = base + 3 * nc;      // 3 nc
-  int32_t* occ = reinterpret_cast<int32_t*>(drift + 3 * nc);  // nc
+  // per process p = c z + j (carrier c, slot j of its site): the values of
+  // its rate that depend only on the site, refreshed when c hops
+  double* pdphi = drift + 3 * nc;     // phi_L[n] - phi_L[s]
+  double* ptq = pdphi + nproc;        // q_c tself[s, j]
+  double* pdE = ptq + nproc;          // E[n] - E[s]
+  double* plam = pdE + nproc;         // lambda
+  double* pvab = plam + nproc;        // V_ab
+  double* pshift = pvab + nproc;      // field . hop vector / 2
+  double* pvec = pshift + nproc;      // hop vector (3 nproc)
+  int32_t* pn = reinterpret_cast<int32_t*>(pvec + 3 * nproc);  // target n
+  int32_t* occ = pn + nproc;          // nc
   __shared__ double sh_t, sh_energy, sh_dg;
   __shared__ int64_t sh_start, sh_end, sh_hops, sh_row_lo, sh_row_hi,
       sh_dp_hi;
-  __shared__ int32_t sh_stop, sh_noselect;
+  __shared__ int32_t sh_stop, sh_noselect, sh_mover;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
+  // the site-dependent part of process p's rate, from the slot tables
+  auto load_process = [&](int p) {
+    const int c = p / z, j = p - c * z;
+    const int s = occ[c];
+    const int64_t sj = int64_t(s) * d.max_z + j;
+    const int n = d.nbr[sj];
+    pn[p] = n;
+    pdphi[p] = d.phiL[n] - d.phiL[s];
+    ptq[p] = d.cq[c] * d.tself[sj];
+    pdE[p] = d.E[n] - d.E[s];
+    plam[p] = d.lam[sj];
+    pvab[p] = d.vab[sj];
+    double shift = 0.0;
+    if (d.field_on)
+      shift = 0.5 * (d.field[0] * d.vec[3 * sj] + d.field[1] * d.vec[3 * sj + 1] +
+                     d.field[2] * d.vec[3 * sj + 2]);
+    pshift[p] = shift;
+    pvec[3 * p] = d.vec[3 * sj];
+    pvec[3 * p + 1] = d.vec[3 * sj + 1];
+    pvec[3 * p + 2] = d.vec[3 * sj + 2];
+  };
   for (int c = tid; c < nc; c += kThreads) occ[c] = d.occ[c];
   for (int k = tid; k < 3 * nc; k += kThreads) {
     disp[k] = d.disp[k];
@@ -112,7 +143,10 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
     sh_hops = d.sc->hops;
     sh_noselect = d.sc->no_select;
     sh_stop = d.sc->finished ? 1 : 0;
+    sh_mover = -1;
   }
+  __syncthreads();
+  for (int p = tid; p < nproc; p += kThreads) load_process(p);
   __syncthreads();
   int64_t h = 0;  // hops made in this call
   int stop = sh_stop;
@@ -146,27 +180,23 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
       for (; c2 < nc; ++c2) acc = fma(d.cq[c2], row[occ[c2]], acc);
       psi[r] = acc;
     }
+    // the last hop's mover: its processes reloaded for its new site, by
+    // the last z threads while the gathers above are in flight
+    if (sh_mover >= 0 && tid >= kThreads - z)
+      load_process(sh_mover * z + (kThreads - 1 - tid));
     __syncthreads();
     // ---- B: rates
     for (int p = tid; p < nproc; p += kThreads) {
       const int c = p / z;
       const int j = p - c * z;
-      const int s = occ[c];
-      const int64_t sj = int64_t(s) * d.max_z + j;
-      const int n = d.nbr[sj];
       const double qs = d.cq[c];
-      const double term01 = (d.phiL[n] - d.phiL[s]) +
-                            (psi[c * (1 + z) + 1 + j] - psi[c * (1 + z)]);
-      const double term02 = qs * d.tself[sj];
-      const double g0 = 2 * qs * (term01 + term02) + (d.E[n] - d.E[s]);
-      const double lam = d.lam[sj];
-      double shift = 0.0;
-      if (d.field_on)
-        shift = 0.5 * (d.field[0] * d.vec[3 * sj] +
-                       d.field[1] * d.vec[3 * sj + 1] +
-                       d.field[2] * d.vec[3 * sj + 2]);
-      const double gs = (((lam + g0) * (lam + g0) / (4 * lam)) - d.vab[sj]) -
-                        shift;
+      const double term01 =
+          pdphi[p] + (psi[c * (1 + z) + 1 + j] - psi[c * (1 + z)]);
+      const double term02 = ptq[p];
+      const double g0 = 2 * qs * (term01 + term02) + pdE[p];
+      const double lam = plam[p];
+      const double gs = (((lam + g0) * (lam + g0) / (4 * lam)) - pvab[p]) -
+                        pshift[p];
       kk[p] = d.nu * pow(kE, -gs / d.kT);
       dg[p] = g0;
     }
@@ -203,10 +233,8 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
       if (lane == 0) {
         if (!mask) ++sh_noselect;
         const int c = sel / z;
-        const int j = sel - c * z;
-        const int s_old = occ[c];
-        const int64_t sj = int64_t(s_old) * d.max_z + j;
-        const int s_new = d.nbr[sj];
+        const int s_new = pn[sel];
+        const double* v = pvec + 3 * sel;
         double t = sh_t;
         int64_t end;
         if (u2 > 0.0) {
@@ -223,13 +251,14 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
         // delg_0 path: [start, end) clipped to P (numpy slice semantics)
         sh_dp_hi = min(end, d.P);
         occ[c] = s_new;
-        disp[3 * c] += d.vec[3 * sj];
-        disp[3 * c + 1] += d.vec[3 * sj + 1];
-        disp[3 * c + 2] += d.vec[3 * sj + 2];
+        sh_mover = c;
+        disp[3 * c] += v[0];
+        disp[3 * c + 1] += v[1];
+        disp[3 * c + 2] += v[2];
         if (d.field_on) {
-          drift[3 * c] += d.vec[3 * sj] * kk[sel];
-          drift[3 * c + 1] += d.vec[3 * sj + 1] * kk[sel];
-          drift[3 * c + 2] += d.vec[3 * sj + 2] * kk[sel];
+          drift[3 * c] += v[0] * kk[sel];
+          drift[3 * c + 1] += v[1] * kk[sel];
+          drift[3 * c + 2] += v[2] * kk[sel];
         }
         sh_energy += g;
         if (hop_time) hop_time[h] = t;
@@ -394,8 +423,8 @@ int check_problem(pycd_ctx* ctx, const pycd_kmc_problem* pb) {
   if (!pb) return fail(ctx, PYCD_ERR_ARG, "null problem");
   if (pb->n_sites <= 0 || pb->n_sites > (int64_t(1) << 31) - 1)
     return fail(ctx, PYCD_ERR_ARG, "bad n_sites");
-  if (pb->n_carriers <= 0 || pb->z <= 0 || pb->max_z < pb->z)
-    return fail(ctx, PYCD_ERR_ARG, "bad carrier/slot counts");
+  if (pb->n_carriers <= 0 || pb->z <= 0 || pb->z > 64 || pb->max_z < pb->z)
+    return fail(ctx, PYCD_ERR_ARG, "bad carrier/slot counts (z <= 64)");
   if (!pb->table || !pb->lattice_q || !pb->nbr_site || !pb->nbr_vec ||
       !pb->nbr_lambda || !pb->nbr_vab || !pb->site_energy || !pb->carrier_q)
     return fail(ctx, PYCD_ERR_ARG, "null table pointer");
@@ -415,9 +444,9 @@ extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
   DeviceGuard guard(ctx->device);
   const int64_t N = pb->n_sites;
   const int nc = pb->n_carriers, z = pb->z, mz = pb->max_z;
-  const size_t smem = sizeof(double) * (size_t(nc) * (1 + z) + 2 * size_t(nc) * z +
+  const size_t smem = sizeof(double) * (size_t(nc) * (1 + z) + 11 * size_t(nc) * z +
                                         9 * size_t(nc)) +
-                      sizeof(int32_t) * nc + 16;
+                      sizeof(int32_t) * (size_t(nc) * z + nc) + 16;
   if (smem > 60 * 1024)
     return fail(ctx, PYCD_ERR_ARG,
                 "n_c=%d, z=%d need %zu B of LDS (limit 60 KiB)", nc, z, smem);
