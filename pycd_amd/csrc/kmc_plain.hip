@@ -168,8 +168,17 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
       const int s = occ[c];
       const int x = slot == 0 ? s : d.nbr[int64_t(s) * d.max_z + slot - 1];
       const double* row = d.A + int64_t(x) * d.N;
+      // the row's gathers in flight 16 (then 8) at a time, summed in
+      // carrier order
       double acc = 0.0;
       int c2 = 0;
+      for (; c2 + 16 <= nc; c2 += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = row[occ[c2 + u]];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc = fma(d.cq[c2 + u], v[u], acc);
+      }
       for (; c2 + 8 <= nc; c2 += 8) {
         double v[8];
 #pragma unroll
