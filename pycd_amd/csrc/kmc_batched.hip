@@ -498,9 +498,10 @@ __device__ __forceinline__ void with_count(int n, F&& f) {
 }
 
 // Wavefronts per SIMD the register allocation targets: 4 (16 per CU, every
-// trajectory of a 4096-trajectory batch resident at once, at the price of a
-// few spilled registers) for FP32 rates with at most 4 processes per lane;
-// 2 otherwise (the FP64-rate and large instances would spill heavily).
+// trajectory of a 4096-trajectory batch resident at once, in 128 VGPRs with
+// only the per-trajectory set-up spilling) for FP32 rates with at most 4
+// processes per lane; 2 otherwise (the FP64-rate and large instances would
+// spill heavily in 128).
 #ifndef PYCD_BATCH_WAVES_PER_EU
 #define PYCD_BATCH_WAVES_PER_EU(f32, per) ((f32) && (per) <= 4 ? 4 : 2)
 #endif
