@@ -280,11 +280,16 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
         }
         sh_end = end;
         sh_hops += 1;
+        // loop condition of the reference: continue while end < P
+        sh_stop = (sh_row_hi > sh_row_lo && sh_row_hi >= d.P) ? 1 : 0;
       }
     }
     __syncthreads();
-    // ---- D: path-step outputs on all threads
-    {
+    ++h;
+    // ---- D: path-step outputs on all threads, for a hop that ends one or
+    // more path steps (otherwise nothing is written and no barrier is due:
+    // the next hop's first two barriers order C's shared values)
+    if (sh_row_hi > sh_row_lo) {
       const int64_t lo = sh_row_lo, hi = sh_row_hi;
       const double g = sh_dg;
       for (int64_t r = lo + tid; r < sh_dp_hi; r += kThreads) d.dpath[r] = g;
@@ -298,19 +303,14 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
         }
         for (int64_t r = lo + tid; r < hi; r += kThreads) d.epath[r] = e;
       }
-    }
-    __syncthreads();
-    if (sh_row_hi > sh_row_lo) {
+      __syncthreads();
       for (int k = tid; k < 3 * nc; k += kThreads) {
         base[k] += disp[k];
         disp[k] = 0.0;
       }
-      if (tid == 0) sh_start = sh_row_hi;
+      if (tid == 0) sh_start = hi;
+      __syncthreads();
     }
-    ++h;
-    // loop condition of the reference: continue while end < P
-    if (tid == 0) sh_stop = (sh_row_hi > sh_row_lo && sh_row_hi >= d.P) ? 1 : 0;
-    __syncthreads();
     stop = sh_stop;
     if (stop) code = 0;
   }
