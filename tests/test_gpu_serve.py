@@ -147,3 +147,30 @@ def test_cancel_and_timeout_requeue_rerun_identically():
     srv.close()
     for a, b in zip(ref, jobs):
         assert _result(a) == _result(b)
+
+
+@pytest.mark.parametrize("rates", ["f64", "f32"])
+def test_large_instance_kernel_matches_small(rates):
+    """A pool sized for up to 64 carriers and 7 slots launches the kernel's
+    large instance (8 processes per lane, 8 slots per carrier); the same jobs
+    in a pool sized for 16 carriers and 4 slots launch the small one.  Every
+    result is the same."""
+    rng = np.random.default_rng(3)
+    specs = [dict(edge=int(rng.integers(3, 5)), n_carriers=int(rng.integers(1, 17)),
+                  hops=int(rng.integers(500, 3000)), seed=300 + k,
+                  temp=float(rng.uniform(250, 400)),
+                  field_vpm=float(rng.choice([0.0, 1e6, 5e6])))
+             for k in range(24)]
+    out = []
+    for nc_max, z_max in ((16, 4), (64, 7)):
+        srv = KmcServer(corundum_family, n_slots=32, nc_max=nc_max,
+                        z_max=z_max, rows_capacity=1 << 22, slice_hops=700,
+                        rate_precision=rates, path_rows=50, msd_lags=20)
+        jobs = [Job(k, **s) for k, s in enumerate(specs)]
+        for j in jobs:
+            srv.submit(j)
+        srv.drain()
+        srv.close()
+        out.append([_result(j) for j in jobs])
+    assert all(r[2] > 0 for r in out[0])
+    assert out[0] == out[1]
