@@ -92,3 +92,36 @@ def test_fp32_batch_diffusivity_within_two_sigma():
     assert abs(m_o - m_g) <= 2 * np.hypot(s_o, s_g)
     assert abs(m_o - exact) <= 2 * s_o + 1e-300
     bt.close()
+
+
+def test_fp32_table_batch_diffusivity_within_two_sigma():
+    """The same check with the family's table held in FP32 on the device."""
+    w = workloads.corundum(4)
+    table = w.build_table()
+    run = w.make_run(table)
+    fam = B.Family(run, table)
+    kT = 300.0 * constants.K2AUTEMP
+    rate = B.expected_rate_per_carrier(fam, kT)
+    dt = HOPS / rate / ROWS
+    n_traj = 256
+    occs, words = rng.seeded_starts(
+        37, n_traj, [fam.hop_sites] * n_traj, [1] * n_traj)
+    trajs = [dict(family=0, n_carriers=1, carrier_q=-1.0, field=np.zeros(3),
+                  kT=kT, time_interval=dt, n_path=2 * ROWS + 1,
+                  max_hops=HOPS) for _ in range(n_traj)]
+    bt = B.KmcBatch([fam], "f32", "f32")
+    bt.setup(trajs, occs, words, msd_lags=[LAGS] * n_traj,
+             msd_scale=1 / constants.ANG2BOHR, waves_per_cu=4)
+    bt.run()
+    curves = bt.msd()
+    assert np.all(bt.results()["hops"] == HOPS)
+    bt.close()
+    d_gpu = _d_over_kbt(curves, dt, 300.0)
+    mean, sem = d_gpu.mean(), d_gpu.std() / np.sqrt(n_traj)
+    _, dcm = exact_diffusivity(run.nbr_site, run.nbr_vec, run.nbr_lambda,
+                               run.nbr_vab, run.nbr_count,
+                               np.arange(w.n_sites) % 12, run.material.vn,
+                               run.temp)
+    exact = over_kbt(dcm, 300.0)
+    print(f"GPU FP32-table D/kBT = {mean:.5e} +- {sem:.2e}; exact {exact:.5e}")
+    assert abs(mean - exact) <= 2 * sem
