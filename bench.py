@@ -48,6 +48,8 @@ def parse():
                     help="0: autotune")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-single", action="store_true",
+                    help="CPU baseline on one core only")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=1000)
@@ -317,51 +319,161 @@ def run_precision(args, log):
     return out
 
 
-def cpu_baseline(args, trajs, fams, log):
-    """NumPy oracle on a bounded sample of the batched workload (1 core)."""
+def _oracle_rate(pb, occ, gen, budget):
+    """Steady-state seconds per hop of the NumPy oracle on one trajectory,
+    run for about `budget` seconds: the per-trajectory set-up (initial energy
+    q^T A q) is timed with max_hops=0 and subtracted.  Returns (hops, s)."""
+    from oracle import kmc
+    t0 = time.perf_counter()
+    kmc.run(pb, occ, gen.random, max_hops=0)
+    setup = time.perf_counter() - t0
+    # time per hop from the difference of two short runs (the set-up, in
+    # both, cancels), so that noise in the set-up time cannot shrink it
+    t0 = time.perf_counter()
+    kmc.run(pb, occ, gen.random, max_hops=50)
+    t1 = time.perf_counter()
+    kmc.run(pb, occ, gen.random, max_hops=250)
+    t2 = time.perf_counter()
+    per = max(((t2 - t1) - (t1 - t0)) / 200, 1e-7)
+    n = max(50, int(budget / per))
+    t0 = time.perf_counter()
+    res = kmc.run(pb, occ, gen.random, max_hops=n)
+    return res["hops"], max(time.perf_counter() - t0 - setup, 1e-9)
+
+
+def _oracle_inputs(t, fam, k):
     import random
     from oracle import kmc
-    from pycd_amd.rng import UniformStream
+    run = fam.run
+    pb = kmc.Problem(fam.table, fam.lattice_q, run.nbr_count, run.nbr_site,
+                     run.nbr_vec, run.nbr_lambda, run.nbr_vab,
+                     run.system_relative_energies, [-1.0] * t["n_carriers"],
+                     t["field"], fam.nu, t["kT"], t["time_interval"],
+                     t["n_path"])
+    gen = random.Random()
+    gen.seed(int(k))
+    occ = gen.sample(fam.hop_sites, t["n_carriers"])
+    return pb, occ, gen
+
+
+def cpu_worker(spec, budget):
+    """One worker of the all-cores baseline: the oracle on the trajectory
+    described by the .npz `spec` (its table memory-mapped from the .npy it
+    names), for about `budget` seconds; prints hops and seconds as JSON."""
+    import random
+    from oracle import kmc
+    z = np.load(spec, allow_pickle=False)
+    table = np.load(str(z["table"]), mmap_mode="r")
+    pb = kmc.Problem(table, z["lattice_q"], z["count"], z["nbr"], z["vec"],
+                     z["lam"], z["vab"], z["energy"], z["carrier_q"],
+                     z["field"], float(z["nu"]), float(z["kT"]),
+                     float(z["dt"]), int(z["n_path"]))
+    gen = random.Random()
+    gen.seed(int(z["seed"]))
+    occ = gen.sample([int(v) for v in z["hop_sites"]], int(z["nc"]))
+    hops, el = _oracle_rate(pb, occ, gen, float(budget))
+    print(json.dumps({"hops": int(hops), "seconds": el}), flush=True)
+
+
+def cpu_all_cores(args, trajs, fams, cores, log):
+    """The oracle on `cores` sampled trajectories at once, one worker
+    process (one thread) per core.  Every trajectory of the workload makes
+    the same number of hops, so cores working through it run at
+    cores / mean(time per hop) over the sample."""
+    import subprocess
+    import tempfile
+    rng_ = np.random.default_rng(98)
+    sample = rng_.choice(len(trajs), size=min(cores, len(trajs)),
+                         replace=False)
+    # tables memory-mapped by the workers: kept in memory (/dev/shm) when
+    # the host has it
+    shm = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    with tempfile.TemporaryDirectory(dir=shm) as tmp:
+        tables, procs = {}, []
+        for k in sample:
+            t = trajs[k]
+            f = t["family"]
+            fam = fams[f]
+            if f not in tables:
+                tables[f] = os.path.join(tmp, f"table{f}.npy")
+                np.save(tables[f], np.ascontiguousarray(fam.table))
+            run = fam.run
+            spec = os.path.join(tmp, f"traj{k}.npz")
+            np.savez(spec, table=tables[f], lattice_q=fam.lattice_q,
+                     count=run.nbr_count, nbr=run.nbr_site, vec=run.nbr_vec,
+                     lam=run.nbr_lambda, vab=run.nbr_vab,
+                     energy=run.system_relative_energies,
+                     carrier_q=np.full(t["n_carriers"], -1.0),
+                     field=np.asarray(t["field"]), nu=fam.nu, kT=t["kT"],
+                     dt=t["time_interval"], n_path=t["n_path"], seed=int(k),
+                     hop_sites=np.asarray(fam.hop_sites),
+                     nc=t["n_carriers"])
+            env = dict(os.environ, OMP_NUM_THREADS="1",
+                       OPENBLAS_NUM_THREADS="1")
+            procs.append(subprocess.Popen(
+                [sys.executable, "-c",
+                 f"import bench; bench.cpu_worker({spec!r}, "
+                 f"{args.cpu_seconds!r})"],
+                cwd=str(Path(__file__).resolve().parent), env=env,
+                stdout=subprocess.PIPE, text=True))
+        log(f"cpu all cores: {len(procs)} workers started")
+        per_hop, hops = [], 0
+        for i, p in enumerate(procs):
+            try:
+                out, _ = p.communicate(timeout=4 * args.cpu_seconds + 60)
+            except subprocess.TimeoutExpired:
+                p.kill()
+                p.communicate()
+                log(f"cpu worker {i} timed out; left out")
+                continue
+            if p.returncode != 0:
+                raise RuntimeError(f"cpu worker exited with {p.returncode}")
+            r = json.loads(out.strip().splitlines()[-1])
+            per_hop.append(r["seconds"] / max(r["hops"], 1))
+            hops += r["hops"]
+            log(f"cpu worker {i}: {r['hops']} hops in {r['seconds']:.1f} s")
+    total = len(per_hop) / float(np.mean(per_hop))
+    log(f"cpu all cores: {len(per_hop)} workers, {total:.1f} hops/s")
+    return total, hops, len(per_hop)
+
+
+def cpu_baseline(args, trajs, fams, log):
+    """NumPy oracle on a bounded sample of the batched workload: one core,
+    then one trajectory per host core at once."""
     rng_ = np.random.default_rng(99)
     sample = rng_.choice(len(trajs), size=min(8, len(trajs)), replace=False)
     budget = args.cpu_seconds / len(sample)
     per_hop, done_hops, spent = [], 0, 0.0
     for k in sample:
         t = trajs[k]
-        fam = fams[t["family"]]
-        run = fam.run
-        pb = kmc.Problem(fam.table, fam.lattice_q, run.nbr_count,
-                         run.nbr_site, run.nbr_vec, run.nbr_lambda,
-                         run.nbr_vab, run.system_relative_energies,
-                         [-1.0] * t["n_carriers"], t["field"], fam.nu,
-                         t["kT"], t["time_interval"], t["n_path"])
-        gen = random.Random()
-        gen.seed(int(k))
-        occ = gen.sample(fam.hop_sites, t["n_carriers"])
-        # steady-state hops only: the per-trajectory set-up (initial energy
-        # q^T A q) is timed separately with max_hops=0 and subtracted
-        t0 = time.perf_counter()
-        kmc.run(pb, occ, gen.random, max_hops=0)
-        setup = time.perf_counter() - t0
-        probe = 20
-        t0 = time.perf_counter()
-        kmc.run(pb, occ, gen.random, max_hops=probe)
-        per = max((time.perf_counter() - t0 - setup) / probe, 1e-6)
-        n = max(probe, int(budget / per))
-        t0 = time.perf_counter()
-        res = kmc.run(pb, occ, gen.random, max_hops=n)
-        el = max(time.perf_counter() - t0 - setup, 1e-9)
+        pb, occ, gen = _oracle_inputs(t, fams[t["family"]], k)
+        hops, el = _oracle_rate(pb, occ, gen, budget)
         spent += el
-        done_hops += res["hops"]
-        per_hop.append(el / max(res["hops"], 1))
+        done_hops += hops
+        per_hop.append(el / max(hops, 1))
+        log(f"cpu one core: trajectory {int(k)}, {hops} hops in {el:.1f} s")
     # every trajectory of the workload makes the same number of hops, so the
     # workload rate is 1 / mean(time per hop) over the sampled trajectories
-    value = 1.0 / float(np.mean(per_hop))
-    return {"value": value, "unit": "hops/s", "cores": 1, "kind": "port",
-            "sample": f"{len(sample)} trajectories of the batched workload "
-                      f"(equal hop counts), {done_hops} hops in {spent:.1f} "
-                      f"s, NumPy oracle of Run.do_kmc_steps, 1 thread; "
-                      f"rate = 1 / mean time per hop"}
+    one = 1.0 / float(np.mean(per_hop))
+    out = {"value": one, "unit": "hops/s", "cores": 1, "kind": "port",
+           "single_core": one,
+           "sample": f"{len(sample)} trajectories of the batched workload "
+                     f"(equal hop counts), {done_hops} hops in {spent:.1f} "
+                     f"s, NumPy oracle of Run.do_kmc_steps, 1 thread; "
+                     f"rate = 1 / mean time per hop"}
+    # the host's cores (the box's share: at most 16), one trajectory each
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    if cores > 1 and not args.cpu_single:
+        total, hops, n = cpu_all_cores(args, trajs, fams, cores, log)
+        out.update(value=total, cores=n, sample=(
+            f"{n} trajectories of the batched workload at once, one worker "
+            f"process per core, about {args.cpu_seconds:.0f} s each "
+            f"({hops} hops), NumPy oracle of Run.do_kmc_steps, 1 thread per "
+            f"worker; value = cores / mean steady-state time per hop; "
+            f"single_core = "
+            f"{len(sample)} trajectories one after another on one core "
+            f"({done_hops} hops in {spent:.1f} s)"))
+    return out
 
 
 def main():
