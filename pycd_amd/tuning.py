@@ -2,7 +2,9 @@
 
 Two knobs decide how trajectories map onto the chip:
   * the launch order of the work queue -- longest expected work first
-    (LPT), so that the last wavefronts to finish carry short trajectories;
+    (LPT).  The library deals this order out to one bin per SIMD in
+    serpentine order, so each SIMD gets a mix of long and short
+    trajectories, and the wavefronts sharing a SIMD pace each other;
   * waves per CU of the persistent grid -- more waves hide more gather
     latency but split the CU's issue slots and LDS.
 `autotune_waves` times short runs of the real batch at each candidate and
