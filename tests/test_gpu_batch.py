@@ -11,10 +11,10 @@ from tests.systems import oracle_problem
 pytestmark = pytest.mark.gpu
 
 
-def _families(sizes, **kw):
+def _families(sizes, builder=workloads.corundum, **kw):
     fams, runs = [], []
     for n in sizes:
-        w = workloads.corundum(n, **kw)
+        w = builder(n, **kw)
         table = w.build_table()
         run = w.make_run(table)
         fams.append(B.Family(run, table))
@@ -144,3 +144,31 @@ def test_queue_bins_leave_results_unchanged():
             assert np.array_equal(res[key], ref[key]), key
         for a, b in zip(rows, ref_rows):
             assert np.array_equal(a, b)
+
+
+def test_fp64_batch_monoclinic_matches_oracle_per_trajectory():
+    """The serving form's monoclinic lattice (C-centred, 4 sites per cell,
+    two hop slots) in the batched FP64 mode, against the oracle."""
+    fams, runs = _families([4, 5], builder=workloads.monoclinic,
+                           t_final=2e-8)
+    specs = [(0, 1, 0.0, 300.0), (1, 7, 1e6, 320.0), (0, 12, 5e6, 280.0),
+             (1, 3, 0.0, 390.0)]
+    seed, max_hops = 13, 1200
+    trajs, occs, words = _mixed_batch(fams, runs, specs, seed, max_hops)
+    bt = B.KmcBatch(fams, "f64", "f64")
+    bt.setup(trajs, occs, words, waves_per_cu=2)
+    bt.run()
+    res = bt.results()
+    for k, spec in enumerate(specs):
+        ora = _oracle(runs, spec, trajs[k], seed, k, len(specs), max_hops)
+        assert res["hops"][k] == ora["hops"] == max_hops
+        # the clock sums log(u2) / k_tot; a rounding difference d in an
+        # energy moves a rate by d / kT relative (1e-15 Ha at ~1e-3 Ha is
+        # 1e-12), so the final clocks agree to 1e-10 here, not 1e-12
+        assert abs(res["t"][k] - ora["time_data"][-1]) <= \
+            1e-10 * ora["time_data"][-1], k
+        assert np.max(np.abs(bt.rows(k) - ora["unwrapped"])) < 1e-8, k
+        scale = max(1.0, abs(ora["energy_now"]))
+        assert abs(res["energy"][k] - ora["energy_now"]) < 1e-9 * scale, k
+        assert res["start"][k] == ora["start"]
+    bt.close()
