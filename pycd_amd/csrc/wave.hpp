@@ -87,10 +87,13 @@ __device__ inline double fold16(double a, double b) {
 // each row.  (Four wave_sum calls would take four full reductions.)
 __device__ inline void wave_sum4(const double* v, double* out) {
   double x = fold16(fold32(v[0], v[2]), fold32(v[1], v[3]));
-  x += dpp_f64<kDppRowRor + 8, 0xf, false>(x);
-  x += dpp_f64<kDppRowRor + 4, 0xf, false>(x);
-  x += dpp_f64<kDppRowRor + 2, 0xf, false>(x);
-  x += dpp_f64<kDppRowRor + 1, 0xf, false>(x);
+  // a rotation reads a lane of its own row for every lane, so the bound
+  // control (which would zero a missing source) never applies; setting it
+  // spares the zeroed old value of each move
+  x += dpp_f64<kDppRowRor + 8, 0xf, true>(x);
+  x += dpp_f64<kDppRowRor + 4, 0xf, true>(x);
+  x += dpp_f64<kDppRowRor + 2, 0xf, true>(x);
+  x += dpp_f64<kDppRowRor + 1, 0xf, true>(x);
 #pragma unroll
   for (int r = 0; r < 4; ++r) out[r] = read_lane(x, 16 * r);
 }
