@@ -174,3 +174,52 @@ def test_large_instance_kernel_matches_small(rates):
         out.append([_result(j) for j in jobs])
     assert all(r[2] > 0 for r in out[0])
     assert out[0] == out[1]
+
+
+def test_served_jobs_equal_the_batched_trajectories():
+    """A job served in slices through the pool gives, bit for bit, the
+    result of the same trajectory run in one batched launch."""
+    from pycd_amd import batch as B, constants
+    from pycd_amd.rng import seeded_starts
+    rng = np.random.default_rng(9)
+    specs = [dict(edge=int(rng.integers(3, 5)), n_carriers=int(rng.integers(1, 13)),
+                  hops=int(rng.integers(800, 2500)), seed=500 + k,
+                  temp=float(rng.uniform(260, 380)),
+                  field_vpm=float(rng.choice([0.0, 2e6])))
+             for k in range(10)]
+    srv = KmcServer(corundum_family, n_slots=16, nc_max=16, rows_capacity=1 << 22,
+                    slice_hops=311, rate_precision="f64", path_rows=50,
+                    msd_lags=20)
+    jobs = [Job(k, **s) for k, s in enumerate(specs)]
+    for j in jobs:
+        srv.submit(j)
+    srv.drain()
+    n_path = srv.n_path
+    srv.close()
+    edges = sorted({s["edge"] for s in specs})
+    fams = [B.Family(*corundum_family(e)) for e in edges]
+    trajs, occs, words = [], [], []
+    for s in specs:
+        fam = fams[edges.index(s["edge"])]
+        o, w = seeded_starts(s["seed"], 1, [fam.hop_sites], [s["n_carriers"]])
+        kT = s["temp"] * constants.K2AUTEMP
+        field = (np.array([0.0, 0.0, s["field_vpm"] * constants.VPM2AUFIELD])
+                 if s["field_vpm"] else np.zeros(3))
+        rate = s["n_carriers"] * B.expected_rate_per_carrier(fam, kT, field)
+        trajs.append(dict(family=edges.index(s["edge"]),
+                          n_carriers=s["n_carriers"], carrier_q=-1.0,
+                          field=field, kT=kT,
+                          time_interval=s["hops"] / rate / 50,
+                          n_path=n_path, max_hops=s["hops"]))
+        occs.append(o[0])
+        words.append(w[0])
+    bt = B.KmcBatch(fams, "f64", "f64")
+    bt.setup(trajs, occs, np.asarray(words), waves_per_cu=4)
+    bt.run()
+    res = bt.results()
+    for k, j in enumerate(jobs):
+        r = j.result
+        assert (r["t"], r["energy"], r["hops"], r["start"]) == \
+            (res["t"][k], res["energy"][k], res["hops"][k], res["start"][k]), k
+        assert np.array_equal(r["rows"], bt.rows(k)), k
+    bt.close()
