@@ -161,6 +161,16 @@ int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
                     pycd_kmc** out);
 void pycd_kmc_destroy(pycd_kmc* s);
 
+/* Replaces the lattice charges and site energies (n_sites doubles each,
+ * host or device) for the trajectories begun after this call, and forms
+ * phi_lattice and q_lattice^T A q_lattice again; the table stays resident.
+ * A doped trajectory has its own of both: dopant sites carry the dopant's
+ * charge (PyCD/core.py:2545-2550) and sites within a dopant's shells are
+ * shifted in energy (PyCD/core.py:2723-2761).  Ends any running
+ * trajectory: call pycd_kmc_begin next. */
+int pycd_kmc_set_sites(pycd_kmc* s, const double* lattice_q,
+                       const double* site_energy);
+
 /* Starts a trajectory: carriers at `occupancy` (n_c sites), clock 0,
  * energy = ewald_neut + q^T A q (PyCD/core.py:2773-2780). */
 int pycd_kmc_begin(pycd_kmc* s, const int64_t* occupancy, double ewald_neut);

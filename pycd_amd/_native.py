@@ -87,6 +87,7 @@ _SIGNATURES = {
                                   C.POINTER(C.c_void_p)]),
     "pycd_kmc_destroy": (None, [_P]),
     "pycd_kmc_begin": (C.c_int, [_P, _P, C.c_double]),
+    "pycd_kmc_set_sites": (C.c_int, [_P, _P, _P]),
     "pycd_kmc_advance": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P,
                                    C.POINTER(C.c_int64)]),
     "pycd_kmc_read": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
@@ -361,6 +362,17 @@ class KmcSession:
             self.close()
         except Exception:
             pass
+
+    def set_sites(self, lattice_q, site_energy):
+        """Lattice charges and site energies of the next trajectories."""
+        n = self.pb.table.shape[0]
+        q = _f64(lattice_q)
+        e = _f64(site_energy)
+        if q.shape != (n,) or e.shape != (n,):
+            raise ValueError(f"lattice_q and site_energy must have {n} "
+                             f"entries")
+        _check(self.ctx, lib().pycd_kmc_set_sites(self.h, _ptr(q), _ptr(e)),
+               "pycd_kmc_set_sites")
 
     def begin(self, occupancy, ewald_neut=0.0):
         occ = np.ascontiguousarray(occupancy, dtype=np.int64)

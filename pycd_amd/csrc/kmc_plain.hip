@@ -442,6 +442,24 @@ int check_problem(pycd_ctx* ctx, const pycd_kmc_problem* pb) {
   return PYCD_OK;
 }
 
+// phi_lattice = A q_lattice and q_lattice^T A q_lattice from the resident
+// table and lattice charges.
+hipError_t lattice_terms(pycd_kmc* s) {
+  hipStream_t st = s->ctx->stream;
+  const int64_t N = s->pb.n_sites;
+  const int blocks = int(std::min<int64_t>((N + 3) / 4, 8192));
+  row_dot_kernel<<<blocks, 256, 0, st>>>(s->A.ptr, s->qL.ptr, N, s->phiL.ptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  dot_kernel<<<1, 256, 0, st>>>(s->qL.ptr, s->phiL.ptr, N, s->tmp.ptr);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(&s->qLAqL, s->tmp.ptr, sizeof(double),
+                     hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
+
 }  // namespace
 
 extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
@@ -536,21 +554,29 @@ extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
   S_HIP(upload(s->vab.ptr, pb->nbr_vab, NM, st));
   S_HIP(upload(s->E.ptr, pb->site_energy, N, st));
   S_HIP(upload(s->cq.ptr, pb->carrier_q, nc, st));
-  {
-    const int blocks = int(std::min<int64_t>((N + 3) / 4, 8192));
-    row_dot_kernel<<<blocks, 256, 0, st>>>(s->A.ptr, s->qL.ptr, N, s->phiL.ptr);
-    S_HIP(hipGetLastError());
-    self_term_kernel<<<int((NM + 255) / 256), 256, 0, st>>>(s->A.ptr, s->nbr.ptr,
-                                                           N, mz, s->tself.ptr);
-    S_HIP(hipGetLastError());
-    dot_kernel<<<1, 256, 0, st>>>(s->qL.ptr, s->phiL.ptr, N, s->tmp.ptr);
-    S_HIP(hipGetLastError());
-    S_HIP(hipMemcpyAsync(&s->qLAqL, s->tmp.ptr, sizeof(double),
-                         hipMemcpyDeviceToHost, st));
-    S_HIP(hipStreamSynchronize(st));
-  }
+  self_term_kernel<<<int((NM + 255) / 256), 256, 0, st>>>(s->A.ptr, s->nbr.ptr,
+                                                         N, mz, s->tself.ptr);
+  S_HIP(hipGetLastError());
+  S_HIP(lattice_terms(s));
 #undef S_HIP
   *out = s;
+  return PYCD_OK;
+}
+
+extern "C" int pycd_kmc_set_sites(pycd_kmc* s, const double* lattice_q,
+                                  const double* site_energy) {
+  if (!s) return PYCD_ERR_ARG;
+  pycd_ctx* ctx = s->ctx;
+  if (!lattice_q || !site_energy)
+    return fail(ctx, PYCD_ERR_ARG, "null lattice_q or site_energy");
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = ctx->stream;
+  const int64_t N = s->pb.n_sites;
+  // the next trajectory starts from these sites: a running one is ended
+  s->begun = false;
+  PYCD_HIP(ctx, upload(s->qL.ptr, lattice_q, N, st));
+  PYCD_HIP(ctx, upload(s->E.ptr, site_energy, N, st));
+  PYCD_HIP(ctx, lattice_terms(s));
   return PYCD_OK;
 }
 

@@ -49,3 +49,26 @@ def oracle_problem(run, table=None, field=None):
         if field is None else field,
         run.material.vn, run.temp, run.time_interval,
         int(run.t_final / run.time_interval) + 1)
+
+
+DOPING_SHIFTS = [0.25, -0.05, 0.01]  # eV at hop shells 0, 1, 2 of a dopant
+
+
+def doped_run(ex, num, kind="random", sep=1, initiation="yes", manual=None,
+              n_traj=3, charge=4.0):
+    """The example's Run with Ti substituting Fe (charge `charge`) at `num`
+    sites, site energies shifted by DOPING_SHIFTS around each dopant."""
+    sim = ex.sim
+    spec = {"doping_element_map": ["Fe:Ti"],
+            "charge": {"full": {"Ti": charge}},
+            "num_dopants": [num], "insertion_type": [kind],
+            "min_shell_separation": [sep], "max_attempts": 20,
+            "site_charge_initiation": [initiation],
+            "dopant_site_indices": [manual or []]}
+    energies = {"class_index": sim["relative_energies"]["class_index"],
+                "doping": {"Fe": [DOPING_SHIFTS]}}
+    return Run(ex.system, ex.table, sim["temp"], sim["ion_charge_type"],
+               sim["species_charge_type"], n_traj, sim["t_final"],
+               sim["time_interval"], np.array(sim["species_count"]),
+               sim["initial_occupancy"], energies, sim["external_field"],
+               spec)
