@@ -132,6 +132,13 @@ def run_batched(args, log):
         sel = (ncs >= lo) & (ncs <= hi)
         log(f"  n_c {lo:2d}-{hi:2d}: busy mean {busy[sel].mean() * 1e3:.1f} ms,"
             f" max {busy[sel].max() * 1e3:.1f} ms")
+    # load per SIMD: how many trajectories each ran, and its longest busy time
+    keys, per_simd = np.unique(res["simd"], return_counts=True)
+    for c in np.unique(per_simd):
+        on = np.isin(res["simd"], keys[per_simd == c])
+        log(f"  {int((per_simd == c).sum())} SIMDs ran {c} trajectories: "
+            f"busy mean {busy[on].mean() * 1e3:.1f} ms, "
+            f"max {busy[on].max() * 1e3:.1f} ms")
     resident = min(wpc, bt.resident_per_cu())
     from pycd_amd import _native
     cus = _native.device_cus()

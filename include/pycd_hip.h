@@ -264,6 +264,9 @@ typedef struct {
   int32_t no_select;          /* hops where rounding left no cum > u1       */
   int32_t done;
   double busy_s;              /* device time its wavefront spent on it      */
+  int32_t simd;               /* SIMD that ran it last: XCC << 10 | SE << 7 |
+                                 SH << 6 | CU << 2 | SIMD (from HW_ID)       */
+  int32_t pad0;
 } pycd_kmc_traj_result;
 
 typedef struct pycd_batch pycd_batch;

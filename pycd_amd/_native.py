@@ -65,7 +65,8 @@ class TrajResultC(C.Structure):
     _fields_ = [("t", C.c_double), ("energy", C.c_double),
                 ("hops", C.c_int64), ("start", C.c_int64),
                 ("no_select", C.c_int32), ("done", C.c_int32),
-                ("busy_s", C.c_double)]
+                ("busy_s", C.c_double), ("simd", C.c_int32),
+                ("pad0", C.c_int32)]
 
 
 

@@ -180,7 +180,7 @@ class KmcBatch:
                "pycd_batch_results")
         out = {k: np.array([getattr(r, k) for r in res])
                for k in ("t", "energy", "hops", "start", "no_select", "done",
-                         "busy_s")}
+                         "busy_s", "simd")}
         out["drift"] = drift
         return out
 

@@ -93,7 +93,7 @@ struct TrajOut {
   int32_t no_select;
   int32_t done;      // 1: the trajectory reached max_hops or filled its path
   int32_t started;   // 1: state arrays hold a run in progress
-  int32_t pad0;
+  int32_t simd;      // SIMD key (XCC, SE, CU, SIMD) of its last wavefront
   double busy;       // wall-clock ticks the trajectory's wavefront spent on it
 };
 
@@ -140,6 +140,7 @@ constexpr int kVecStride = 3 * 3 * kMaxCarriers;
 constexpr int kMaxSubSlots = 256;  // n_unit z of a family
 constexpr int kMaxCells = 255;     // cells per axis, sites per cell
 constexpr double kEulerE = 2.718281828459045;
+constexpr double kLog2E = 1.4426950408889634;  // log2(e)
 
 // Values every lane holds identically, moved to scalar registers.
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -614,13 +615,16 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     }
     __syncthreads();
     double t_clk, energy;
-    int64_t start, hops;
+    // path rows are int32 (the host keeps n_path below 2^31)
+    const int32_t P = int32_t(td.P);
+    int32_t start;
+    int64_t hops;
     int no_select;
     if (resume) {
       for (int r = lane; r < R; r += 64) psi[r] = spsi[r];
       t_clk = o0.t;
       energy = o0.energy;
-      start = o0.start;
+      start = int32_t(o0.start);
       hops = o0.hops;
       no_select = o0.no_select;
     } else {
@@ -648,9 +652,11 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     if (a.refresh > 0)
       next_refresh =
           hops == 0 ? a.refresh : (hops + a.refresh - 1) / a.refresh * a.refresh;
-    const float nikTf = float(-1.0 / td.kT);
+    // FP32 rates: nu exp(-gs / kT) = 2^(gs (-log2(e) / kT) + log2(nu)),
+    // one FMA ahead of v_exp_f32
+    const float exp_scale = float(-kLog2E / td.kT);
+    const float exp_bias = float(log2(a.nu));
     const double inv_dt = 1.0 / td.dt;
-    const float nuf = float(a.nu);
 
     // Uniforms of the coming hops, one hop per lane: at a refill (mti = m)
     // lane l takes u1 and log(u2) (-inf for u2 = 0) of the hop that draws
@@ -662,12 +668,17 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     // instead of every lane taking one per hop.
     double c_u1 = 0.0, c_lu2 = 0.0;
     int c_next = 0, c_n = 0;
-    // hops left to run in this launch, as pacing counts them
-    auto hops_left = [&]() {
-      int64_t h = td.max_hops > 0 ? td.max_hops - hops : INT32_MAX;
+    // Hops this launch may still make: to max_hops (which finishes the
+    // trajectory) and to the end of the slice (which pauses it).  The hop
+    // loop counts them down in 32 bits and asks again when they run out, so
+    // a budget above 2^31 - 1 is taken in parts.
+    auto budget = [&]() {
+      int64_t h = INT64_MAX;
+      if (td.max_hops > 0) h = td.max_hops - hops;
       if (a.slice_hops > 0) h = min(h, a.slice_hops - slice_done);
-      return uint32_t(h < INT32_MAX ? h : INT32_MAX);
+      return h;
     };
+    uint32_t left = 0;
     for (;;) {
       // Conditions on the lane and on the trajectory's sizes are formed
       // again each hop from these copies, which the compiler cannot move
@@ -682,16 +693,19 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         pd[i] = pdesc[i];
         asm volatile("" : "+v"(pd[i]));
       }
-      if (td.max_hops > 0 && hops >= td.max_hops) {
-        finished = true;
-        break;
+      if (left == 0) {
+        asm volatile("");  // the 64-bit budget only when the count runs out
+        const int64_t b = budget();
+        if (b <= 0) {
+          finished = td.max_hops > 0 && hops >= td.max_hops;
+          break;
+        }
+        left = uint32_t(min<int64_t>(b, INT32_MAX));
       }
-      if (a.slice_hops > 0 && slice_done >= a.slice_hops) break;
+      --left;
       ++slice_done;
-      if (a.progress && (hops & 63) == 0) {
-        asm volatile("");  // keeps hops_left() from being computed every hop
-        pace(a.progress, pslot, hops_left(), lane);
-      }
+      if (a.progress && (uint32_t(hops) & 63u) == 0)
+        pace(a.progress, pslot, left + 1, lane);
       if (hops == next_refresh) {
         next_refresh += a.refresh;
         // bound the rounding carried by the incremental updates
@@ -742,22 +756,24 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
           const PTab<kF32Rate>& e = ptab[pinfo[p] >> 24];
           const double dpsi =
               psi[(d >> 19) & 0x3ffu] - psi[(d >> 9) & 0x3ffu];
-          double g0, k;
+          double g0;
           if constexpr (kF32Rate) {
             g0 = e.g0s + 2 * q * dpsi;
             const float gf = float(g0);
             const float gs = (e.lam + gf) * (e.lam + gf) * e.i4l - e.vs;
-            k = double(nuf * __expf(gs * nikTf));
+            const float kf =
+                __builtin_amdgcn_exp2f(fmaf(gs, exp_scale, exp_bias));
+            kv[i] = double(real ? kf : 0.0f);
           } else {
             g0 = 2 * q * ((e.dphi + dpsi) + e.tq) + e.dE;
             const double lam = e.lam;
             const double gs =
                 (((lam + g0) * (lam + g0) / (4 * lam)) - e.vab) - e.shift;
-            k = a.nu * pow(kEulerE, -gs / td.kT);
+            const double k = a.nu * pow(kEulerE, -gs / td.kT);
+            kv[i] = real ? k : 0.0;
           }
-          kv[i] = real ? k : 0.0;
           gv[i] = g0;
-          ksum += kv[i];
+          ksum = i == 0 ? kv[0] : ksum + kv[i];
         }
       });
       // ---- FP64 prefix and selection.  FP64 rates: the prefix of the
@@ -823,7 +839,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       const double gsel = read_lane(g_found, src);
       const double ksel = read_lane(k_found, src);
       // ---- clock
-      int64_t end;
+      int32_t end;
       if (lu2 > -INFINITY) {  // u2 > 0
         t_clk -= lu2 / k_tot;
         // path step int(t / dt) of the correctly rounded quotient: below
@@ -836,11 +852,11 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         } else {
           asm volatile("");  // the division only where it is needed
           const double qq = t_clk / td.dt;
-          end = qq < 9.0e18 ? int64_t(qq) : td.P;
+          end = qq < double(P) ? int32_t(qq) : P;
         }
       } else {
         t_clk = INFINITY;
-        end = td.P;
+        end = P;
       }
       energy += gsel;
       // ---- one round of independent loads for the whole update
@@ -938,14 +954,14 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       ++hops;
       bool full = false;
       if (end >= start + 1) {
-        if (end >= td.P) end = td.P;
+        if (end >= P) end = P;
         for (int k = lane_h; k < w; k += 64) {
           const double v = sbase[k] + swap_f64(sdisp + k, 0.0);
-          for (int64_t r = start; r < end; ++r) rows[r * w + k] = v;
+          for (int32_t r = start; r < end; ++r) rows[int64_t(r) * w + k] = v;
           sbase[k] = v;
         }
         start = end;
-        full = end >= td.P;
+        full = end >= P;
       }
       __syncthreads();
       if (full) {
@@ -975,7 +991,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       o.no_select = no_select;
       o.done = finished ? 1 : 0;
       o.started = 1;
-      o.pad0 = 0;
+      o.simd = int32_t(simd_key());
       o.busy = (resume ? o0.busy : 0.0) + double(wall_clock64() - w_start);
       a.out[tr] = o;
     }
