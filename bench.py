@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-single", action="store_true",
                     help="CPU baseline on one core only")
+    ap.add_argument("--no-handover", action="store_true",
+                    help="batched: no hand-over of trajectories to idle "
+                         "wavefronts")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=1000)
@@ -113,6 +116,7 @@ def run_batched(args, log):
                                            log=log)
     bt.setup(trajs, occs, words, order=order, msd_lags=lags,
              msd_scale=1 / constants.ANG2BOHR, waves_per_cu=wpc)
+    bt.set_handover(not args.no_handover)
     for _ in range(args.warmup):
         bt.run(use_graph=True)
     times = []
@@ -126,6 +130,7 @@ def run_batched(args, log):
     # kernel-only timing (no graph) for the roofline of the dominant kernel
     bt.run(use_graph=False)
     bt_k = bt.kernel_ms()
+    log(f"  {bt.handovers()} trajectories handed over to idle wavefronts")
     busy = res["busy_s"]
     ncs = np.array([t["n_carriers"] for t in trajs])
     for lo, hi in ((1, 16), (17, 32), (33, 48), (49, 64)):

@@ -289,6 +289,14 @@ int pycd_batch_setup(pycd_batch* b, const pycd_kmc_traj* trajs,
                      const int64_t* msd_lags, double msd_scale,
                      int32_t waves_per_cu);
 
+/* Hand-over of running trajectories to idle wavefronts in a launch with
+ * SIMD bins (default on).  SIMDs of one chip run the same work at speeds up
+ * to ~8% apart; with it, a trajectory with hops to spare moves, at a hop
+ * boundary and with its exact state, to a wavefront whose SIMD ran out of
+ * work.  Results are the same either way.  No reference counterpart (the
+ * reference runs trajectories one after another, PyCD/core.py:2668). */
+int pycd_batch_set_handover(pycd_batch* b, int32_t on);
+
 /* Runs every trajectory (and the MSD reduction), optionally through a
  * captured hipGraph; *elapsed_ms = device time between events around it. */
 int pycd_batch_run(pycd_batch* b, int32_t use_graph, double* elapsed_ms);
@@ -325,6 +333,9 @@ int pycd_batch_slot_msd(pycd_batch* b, const int64_t* slots, int64_t n,
 /* Device time of the KMC kernel alone (no MSD, no resets) in the last
  * pycd_batch_run without a graph, or the last pycd_batch_step. */
 int pycd_batch_kernel_ms(const pycd_batch* b, double* ms);
+/* Hand-overs claimed in the last launch (0 without SIMD bins or with
+ * hand-over off).  No reference counterpart. */
+int pycd_batch_handovers(pycd_batch* b, int64_t* n);
 /* Wavefronts of the KMC kernel that one CU holds at once for the current
  * set-up (registers and LDS); a grid beyond that many per CU queues. */
 int pycd_batch_resident(const pycd_batch* b, int32_t* per_cu);

@@ -103,11 +103,13 @@ _SIGNATURES = {
                                    C.c_void_p, C.c_double, C.c_int32]),
     "pycd_batch_run": (C.c_int, [C.c_void_p, C.c_int32,
                                  C.POINTER(C.c_double)]),
+    "pycd_batch_set_handover": (C.c_int, [C.c_void_p, C.c_int32]),
     "pycd_batch_results": (C.c_int, [C.c_void_p, C.POINTER(TrajResultC),
                                      C.c_void_p]),
     "pycd_batch_rows": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
     "pycd_batch_msd": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pycd_batch_kernel_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "pycd_batch_handovers": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "pycd_batch_resident": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "pycd_batch_add_family": (C.c_int, [_P, C.POINTER(FamilyC),
                                         C.POINTER(C.c_int32)]),

@@ -153,6 +153,12 @@ class KmcBatch:
         self.msd_lags = lags
         return self
 
+    def set_handover(self, on):
+        """Hand running trajectories over to idle wavefronts (default on
+        for launches with SIMD bins); results are the same either way."""
+        _check(self.ctx, lib().pycd_batch_set_handover(self.h, 1 if on else 0),
+               "pycd_batch_set_handover")
+
     def run(self, use_graph=False):
         ms = C.c_double(0.0)
         _check(self.ctx, lib().pycd_batch_run(self.h, 1 if use_graph else 0,
@@ -165,6 +171,13 @@ class KmcBatch:
         _check(self.ctx, lib().pycd_batch_kernel_ms(self.h, C.byref(ms)),
                "pycd_batch_kernel_ms")
         return ms.value
+
+    def handovers(self):
+        """Trajectories handed over to idle wavefronts in the last launch."""
+        n = C.c_int64(0)
+        _check(self.ctx, lib().pycd_batch_handovers(self.h, C.byref(n)),
+               "pycd_batch_handovers")
+        return n.value
 
     def resident_per_cu(self):
         """Wavefronts of the KMC kernel one CU holds for this set-up."""

@@ -116,7 +116,8 @@ struct BatchArgs {
   int32_t n_bins;
   int32_t pad1;
   // remaining hops of the wavefront in each SIMD wave slot (kSimdKeys x 16,
-  // zeroed before the launch), for pacing; NULL: no pacing
+  // zeroed before the launch), for pacing, followed by the hand-over words
+  // (see claim_mailbox); NULL: no pacing and no hand-over
   uint32_t* progress;
   // resumable state (slices): MT words, occupancy, displacement/base/drift
   uint32_t* mts;            // n_traj x 625
@@ -475,6 +476,84 @@ __device__ inline uint32_t pop_ticket(const BatchArgs& a, int& bin) {
   return uint32_t(a.n_traj);
 }
 
+// Hand-over of running trajectories to idle wavefronts (batch launches with
+// SIMD bins: the words after the pacing slots).  SIMDs of the same chip run the same
+// work at speeds up to ~8% apart, so with every trajectory resident from
+// the start the launch would end with the slowest SIMDs.  Instead, a
+// wavefront that finds the queue empty opens a mailbox and waits; a
+// wavefront whose trajectory has hops to spare saves the trajectory's
+// resumable state (exact, as at the end of a slice), puts its index in a
+// waiting mailbox and leaves.  Words of `hand`:
+//   [0] trajectories not yet finished   [1] mailboxes taken by donors
+//   [2] mailboxes opened                [3] mailboxes (0: hand-over off)
+//   [kHandHdr + k] mailbox k: 0 empty, kMailClosed, or trajectory index + 1.
+// Every wait ends: a mailbox is filled only while its donor's trajectory is
+// unfinished, an empty mailbox is closed (by CAS) once none is, and a wait
+// also gives up after kHandWaitTicks.  A trajectory is handed over at most
+// once (a wavefront never passes on what it received), so at most
+// grid + n_traj mailboxes are opened.
+constexpr int kHandHdr = 4;
+constexpr uint32_t kMailClosed = 0xffffffffu;
+constexpr uint32_t kHandMinHops = 2048;          // hops left to be worth it
+constexpr uint64_t kHandWaitTicks = 2000000000;  // 20 s at 100 MHz
+
+__device__ inline uint32_t* hand_words(uint32_t* progress) {
+  return progress + kSimdKeys * 16;
+}
+
+// Mailbox index of a waiting wavefront taken for a hand-over, or -1
+// (lane 0 only).
+__device__ inline int claim_mailbox(uint32_t* hand) {
+  const uint32_t cap = hand[3];
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    const uint32_t head = __hip_atomic_load(hand + 1, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t open = min(__hip_atomic_load(hand + 2, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT),
+                              cap);
+    if (head >= open) return -1;
+    uint32_t expect = head;
+    if (__hip_atomic_compare_exchange_strong(
+            hand + 1, &expect, head + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT))
+      return int(head);
+  }
+  return -1;
+}
+
+// Opens a mailbox and waits for a trajectory: its index, or -1 when none
+// is left to hand over or hand-over is off (lane 0 only).
+__device__ inline int wait_for_trajectory(uint32_t* hand) {
+  const uint32_t cap = hand[3];
+  if (cap == 0u) return -1;
+  const uint32_t k = atomicAdd(hand + 2, 1u);
+  if (k >= cap) return -1;
+  uint32_t* box = hand + kHandHdr + k;
+  const uint64_t t0 = wall_clock64();
+  // Relaxed polls: an acquire here would invalidate the caches of the
+  // wavefronts still running on every look; the caller takes one acquire
+  // fence once a trajectory has arrived.
+  for (uint32_t it = 0;; ++it) {
+    const uint32_t v =
+        __hip_atomic_load(box, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v != 0u) return int(v - 1u);
+    // the shared count only every 8th look, to keep the waiters off it
+    if ((it & 7u) == 7u &&
+        (__hip_atomic_load(hand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             0u ||
+         wall_clock64() - t0 > kHandWaitTicks)) {
+      uint32_t expect = 0u;
+      if (__hip_atomic_compare_exchange_strong(
+              box, &expect, kMailClosed, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+              __HIP_MEMORY_SCOPE_AGENT))
+        return -1;
+      return int(expect - 1u);  // filled meanwhile
+    }
+    __builtin_amdgcn_s_sleep(127);
+    __builtin_amdgcn_s_sleep(127);
+  }
+}
+
 // Marks the hardware key of every SIMD that runs one of its wavefronts in
 // `seen` (kSimdKeys bits).  Each wavefront sleeps `spin` x ~8k cycles
 // after, so that a grid of several wavefronts per SIMD occupies all of
@@ -526,15 +605,45 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     const int b = a.key_bin[simd_key()];
     bin = uni(b >= 0 ? b : int(blockIdx.x % uint32_t(a.n_bins)));
   }
+  int again = -1;  // a trajectory whose hand-over fell through
   for (;;) {
-    uint32_t ti = 0;
-    if (lane == 0) ti = pop_ticket(a, bin);
-    ti = uni(uint32_t(__shfl(int(ti), 0, 64)));
-    bin = uni(__shfl(bin, 0, 64));
-    if (ti >= uint32_t(a.n_traj)) break;
-    const int tr = uni(a.order[ti]);
+    int tr;
+    bool stolen = false;  // handed over by another wavefront
+    uint32_t ti = uint32_t(a.n_traj);
+    if (again < 0) {
+      if (lane == 0) ti = pop_ticket(a, bin);
+      ti = uni(uint32_t(__shfl(int(ti), 0, 64)));
+      bin = uni(__shfl(bin, 0, 64));
+    }
+    if (again >= 0) {
+      tr = again;  // resumed from the state saved for the hand-over
+      again = -1;
+    } else if (ti < uint32_t(a.n_traj)) {
+      tr = uni(a.order[ti]);
+    } else {
+      if (!a.progress) break;
+      // wait for a hand-over only on a SIMD that has run out of work, so
+      // that a trajectory moves from a busy SIMD to an idle one
+      uint32_t busy_here = 0;
+      if (lane < 16)
+        busy_here = __hip_atomic_load(a.progress + (pslot & ~15u) + lane,
+                                      __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+      if (__ballot(busy_here != 0u)) break;
+      int got = -1;
+      if (lane == 0) got = wait_for_trajectory(hand_words(a.progress));
+      got = uni(__shfl(got, 0, 64));
+      if (got < 0) break;
+      // the donor's saved state, released before its mailbox write
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      tr = got;
+      stolen = true;
+    }
     const TrajOut o0 = a.out[tr];
-    if (uni(o0.done)) continue;
+    if (uni(o0.done)) {
+      if (a.progress && lane == 0) atomicSub(hand_words(a.progress), 1u);
+      continue;
+    }
     const bool resume = uni(o0.started) != 0;
     const uint64_t w_start = wall_clock64();
     const TrajDev td = a.trajs[tr];
@@ -668,6 +777,32 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     // instead of every lane taking one per hop.
     double c_u1 = 0.0, c_lu2 = 0.0;
     int c_next = 0, c_n = 0;
+    // outputs and resumable state: at the trajectory's end, or at a hop
+    // boundary when it is handed over
+    auto save = [&](bool done_now) {
+      for (int k = lane; k < w; k += 64)
+        a.drift[int64_t(tr) * 3 * kMaxCarriers + k] =
+            atomicAdd(sdrift + k, 0.0);
+      for (int k = lane; k < 624; k += 64) mts[k] = mt[k];
+      for (int c = lane; c < nc; c += 64) socc[c] = occ[c];
+      for (int r = lane; r < R; r += 64) spsi[r] = psi[r];
+      if (lane == 0) {
+        mts[624] = uint32_t(mti);
+        TrajOut o;
+        o.t = t_clk;
+        o.energy = energy;
+        o.hops = hops;
+        o.start = start;
+        o.no_select = no_select;
+        o.done = done_now ? 1 : 0;
+        o.started = 1;
+        o.simd = int32_t(simd_key());
+        o.busy = (resume ? o0.busy : 0.0) + double(wall_clock64() - w_start);
+        a.out[tr] = o;
+      }
+      __syncthreads();
+    };
+    int box = -1;  // mailbox of the wavefront taking the trajectory over
     // Hops this launch may still make: to max_hops (which finishes the
     // trajectory) and to the end of the slice (which pauses it).  The hop
     // loop counts them down in 32 bits and asks again when they run out, so
@@ -704,8 +839,14 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       }
       --left;
       ++slice_done;
-      if (a.progress && (uint32_t(hops) & 63u) == 0)
+      if (a.progress && (uint32_t(hops) & 63u) == 0) {
         pace(a.progress, pslot, left + 1, lane);
+        if (!stolen && left >= kHandMinHops) {
+          if (lane_h == 0) box = claim_mailbox(hand_words(a.progress));
+          box = uni(__shfl(box, 0, 64));
+          if (box >= 0) break;  // hand over at this hop boundary
+        }
+      }
       if (hops == next_refresh) {
         next_refresh += a.refresh;
         // bound the rounding carried by the incremental updates
@@ -975,27 +1116,23 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
                            __HIP_MEMORY_SCOPE_AGENT);
       __builtin_amdgcn_s_setprio(0);
     }
-    // ---- outputs and resumable state
-    for (int k = lane; k < w; k += 64)
-      a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = atomicAdd(sdrift + k, 0.0);
-    for (int k = lane; k < 624; k += 64) mts[k] = mt[k];
-    for (int c = lane; c < nc; c += 64) socc[c] = occ[c];
-    for (int r = lane; r < R; r += 64) spsi[r] = psi[r];
-    if (lane == 0) {
-      mts[624] = uint32_t(mti);
-      TrajOut o;
-      o.t = t_clk;
-      o.energy = energy;
-      o.hops = hops;
-      o.start = start;
-      o.no_select = no_select;
-      o.done = finished ? 1 : 0;
-      o.started = 1;
-      o.simd = int32_t(simd_key());
-      o.busy = (resume ? o0.busy : 0.0) + double(wall_clock64() - w_start);
-      a.out[tr] = o;
+    save(box < 0 && finished);
+    if (box >= 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      int handed = 0;
+      if (lane == 0) {
+        uint32_t expect = 0u;
+        handed = __hip_atomic_compare_exchange_strong(
+            hand_words(a.progress) + kHandHdr + box, &expect,
+            uint32_t(tr) + 1u, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // the trajectory lives on in the wavefront it was handed to
+      if (uni(__shfl(handed, 0, 64))) return;
+      again = tr;  // that wavefront stopped waiting: resume it here
+      continue;
     }
-    __syncthreads();
+    if (a.progress && lane == 0) atomicSub(hand_words(a.progress), 1u);
   }
 }
 
@@ -1203,6 +1340,15 @@ __global__ void fill_u32(uint32_t* p, int64_t n, uint32_t v) {
     p[i] = v;
 }
 
+// Hand-over words before a launch: `active` unfinished trajectories, no
+// mailbox taken or opened, `cap` mailboxes (0: off), every mailbox empty.
+__global__ void hand_reset(uint32_t* h, int64_t n, uint32_t active,
+                           uint32_t cap) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    h[i] = i == 0 ? active : (i == 3 ? cap : 0u);
+}
+
 __global__ void reset_outputs(TrajOut* o, int64_t n) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
@@ -1281,8 +1427,12 @@ struct pycd_batch {
   DevBuf<int64_t> d_occ;
   DevBuf<uint32_t> d_mt, d_counter, d_bins;
   DevBuf<int16_t> d_keybin;
-  DevBuf<uint32_t> d_prog;  // pacing slots, one per SIMD wave slot (binned runs)
+  // pacing slots, one per SIMD wave slot, then the hand-over words (binned
+  // runs)
+  DevBuf<uint32_t> d_prog;
   int n_bins = 1;
+  uint32_t hand_cap = 0;  // mailboxes (0 for the serving pool)
+  bool handover = true;
   DevBuf<double> d_rows, d_drift, d_msd;
   DevBuf<TrajOut> d_out;
   DevBuf<uint32_t> d_mts;

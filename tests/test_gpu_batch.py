@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import kmc, msd as omsd, seeding
-from pycd_amd import batch as B, constants, rng, workloads
+from pycd_amd import batch as B, constants, rng, tuning, workloads
 from tests.systems import oracle_problem
 
 pytestmark = pytest.mark.gpu
@@ -142,6 +142,54 @@ def test_queue_bins_leave_results_unchanged():
     for res, rows in out[1:]:
         for key in ("t", "hops", "energy", "start", "drift"):
             assert np.array_equal(res[key], ref[key]), key
+        for a, b in zip(rows, ref_rows):
+            assert np.array_equal(a, b)
+
+
+def test_handover_leaves_results_unchanged():
+    """Short and long trajectories on per-SIMD bins: wavefronts that run out
+    of work take the long ones over at hop boundaries, from their saved
+    state.  Every result equals the run without hand-over, bit for bit,
+    eager and replayed from the graph, and hand-overs did happen."""
+    fams, runs = _families([3], t_final=2e-7, time_interval=1e-11)
+    rs = np.random.default_rng(9)
+    n = 1100
+    specs = [(0, int(nc), float(fv), 300.0)
+             for nc, fv in zip(rs.integers(1, 9, n),
+                               rs.choice([0.0, 1e6], n))]
+    trajs, occs, words = _mixed_batch(fams, runs, specs, 23, 300)
+    long = rs.choice(n, 120, replace=False)
+    for k in long:
+        trajs[k]["max_hops"] = 12000
+    order = tuning.lpt_order(trajs, fams)
+    out = {}
+    for hand in (False, True):
+        bt = B.KmcBatch(fams, "f64", "f32")
+        bt.setup(trajs, occs, words, order=order, waves_per_cu=4)
+        bt.set_handover(hand)
+        bt.run()
+        res = bt.results()
+        out[hand] = (res, [bt.rows(int(k)) for k in long[:6]],
+                     bt.handovers())
+        if hand:
+            bt.run(use_graph=True)
+            bt.run(use_graph=True)
+            out["graph"] = (bt.results(), [bt.rows(int(k)) for k in long[:6]],
+                            bt.handovers())
+        bt.close()
+    ref, ref_rows, none = out[False]
+    assert none == 0
+    # every trajectory ran to one of its limits: max_hops or a full path
+    n_path = np.array([t["n_path"] for t in trajs])
+    max_hops = np.array([t["max_hops"] for t in trajs])
+    assert np.all((ref["hops"] == max_hops) | (ref["start"] == n_path))
+    assert np.mean(ref["hops"][long] == 12000) > 0.9
+    print("hand-overs:", out[True][2], out["graph"][2])
+    for key in (True, "graph"):
+        res, rows, count = out[key]
+        assert count > 0
+        for f in ("t", "hops", "energy", "start", "drift", "no_select"):
+            assert np.array_equal(res[f], ref[f]), (key, f)
         for a, b in zip(rows, ref_rows):
             assert np.array_equal(a, b)
 
