@@ -182,8 +182,11 @@ def run_plain(args, log):
         carrier_q=np.asarray(run.species_charge_list, dtype=np.float64),
         field=run.electric_field, nu=run.material.vn, kT=run.temp,
         time_interval=run.time_interval,
-        n_path=int(run.t_final / run.time_interval) + 1)
+        n_path=int(run.t_final / run.time_interval) + 1,
+        cells=run.system_size)
     sess = _native.KmcSession(pb)
+    log(f"plain: hop loop reads {sess.table_rows()} table rows of "
+        f"{w.n_sites}")
     occs, words = seeded_starts(2, 1, [run.hop_site_lists["electron"]], [16])
     state = (3, tuple(int(v) for v in words[0]), None)
     n = args.plain_hops

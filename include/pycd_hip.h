@@ -150,6 +150,15 @@ typedef struct {
   double kT;                  /* temperature (Hartree)                      */
   double time_interval;       /* path-step width (au time)                  */
   int64_t n_path;             /* P = int(t_final / time_interval) + 1       */
+  /* Optional (all 0: none): the supercell in unit cells and the sites per
+   * cell, sites indexed ((x s_y + y) s_z + z) n_unit + e as PyCD indexes
+   * them.  With it, the hop loop gathers from the n_unit table rows of
+   * cell 0 when the table is above 32 MB (the table of a periodic
+   * supercell is translation-invariant; checked entry by entry at
+   * creation, to 1e-11 of the table's scale, and not used where the check
+   * fails). */
+  int32_t cells[3];
+  int32_t n_unit;
 } pycd_kmc_problem;
 
 typedef struct pycd_kmc pycd_kmc;
@@ -170,6 +179,11 @@ void pycd_kmc_destroy(pycd_kmc* s);
  * trajectory: call pycd_kmc_begin next. */
 int pycd_kmc_set_sites(pycd_kmc* s, const double* lattice_q,
                        const double* site_energy);
+
+/* Table rows the hop loop gathers from: n_unit when the problem gave its
+ * cell shape and the table passed the translation check, n_sites
+ * otherwise.  No reference counterpart. */
+int pycd_kmc_table_rows(const pycd_kmc* s, int64_t* rows);
 
 /* Starts a trajectory: carriers at `occupancy` (n_c sites), clock 0,
  * energy = ewald_neut + q^T A q (PyCD/core.py:2773-2780). */

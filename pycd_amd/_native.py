@@ -40,7 +40,8 @@ class KmcProblemC(C.Structure):
                 ("site_energy", C.c_void_p), ("carrier_q", C.c_void_p),
                 ("field", C.c_double * 3), ("nu", C.c_double),
                 ("kT", C.c_double), ("time_interval", C.c_double),
-                ("n_path", C.c_int64)]
+                ("n_path", C.c_int64), ("cells", C.c_int32 * 3),
+                ("n_unit", C.c_int32)]
 
 
 class FamilyC(C.Structure):
@@ -89,6 +90,7 @@ _SIGNATURES = {
     "pycd_kmc_destroy": (None, [_P]),
     "pycd_kmc_begin": (C.c_int, [_P, _P, C.c_double]),
     "pycd_kmc_set_sites": (C.c_int, [_P, _P, _P]),
+    "pycd_kmc_table_rows": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "pycd_kmc_advance": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P,
                                    C.POINTER(C.c_int64)]),
     "pycd_kmc_read": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
@@ -299,7 +301,10 @@ class KmcProblem:
 
     def __init__(self, table, lattice_q, nbr_count, nbr_site, nbr_vec,
                  nbr_lambda, nbr_vab, site_energy, carrier_q, field, nu, kT,
-                 time_interval, n_path):
+                 time_interval, n_path, cells=None):
+        """`cells`: the supercell size of a periodic lattice (sites in
+        PyCD's cell-major order), so that the hop loop may gather from the
+        table rows of cell 0; None: gather from the full table."""
         self.table = _f64(table)
         n = self.table.shape[0]
         self.lattice_q = _f64(lattice_q)
@@ -321,6 +326,13 @@ class KmcProblem:
                              f"of hop slots (found {sorted(counts)})")
         self.z = counts.pop()
         assert self.lattice_q.shape == (n,)
+        self.cells = (0, 0, 0)
+        self.n_unit = 0
+        if cells is not None:
+            cells = tuple(int(v) for v in cells)
+            n_cells = cells[0] * cells[1] * cells[2]
+            if n_cells > 0 and n % n_cells == 0:
+                self.cells, self.n_unit = cells, n // n_cells
 
 
 class KmcSession:
@@ -350,6 +362,9 @@ class KmcSession:
         c.nu, c.kT = pb.nu, pb.kT
         c.time_interval = pb.time_interval
         c.n_path = pb.n_path
+        for a in range(3):
+            c.cells[a] = pb.cells[a]
+        c.n_unit = pb.n_unit
         h = C.c_void_p()
         _check(self.ctx, lib().pycd_kmc_create(self.ctx, C.byref(c),
                                                C.byref(h)), "pycd_kmc_create")
@@ -365,6 +380,14 @@ class KmcSession:
             self.close()
         except Exception:
             pass
+
+    def table_rows(self):
+        """Table rows the hop loop gathers from (n_unit: cell-0 rows of a
+        periodic supercell; n_sites: the full table)."""
+        n = C.c_int64(0)
+        _check(self.ctx, lib().pycd_kmc_table_rows(self.h, C.byref(n)),
+               "pycd_kmc_table_rows")
+        return n.value
 
     def set_sites(self, lattice_q, site_energy):
         """Lattice charges and site energies of the next trajectories."""

@@ -911,7 +911,10 @@ class Run(object):
             else np.zeros(3),
             nu=self.material.vn, kT=self.temp,
             time_interval=self.time_interval,
-            n_path=int(self.t_final / self.time_interval) + 1)
+            n_path=int(self.t_final / self.time_interval) + 1,
+            # a fully periodic supercell: the loop may gather from the table
+            # rows of cell 0 (checked against the full table on the device)
+            cells=self.system_size if np.all(self.neighbors.pbc) else None)
 
     def do_kmc_steps(self, dst_path, output_data, random_seed, compute_mode,
                      max_hops=None, record=False):

@@ -8,7 +8,9 @@ from pycd_amd import _native, rng
 from tests.systems import oracle_problem
 
 
-def native_problem(run, table):
+def native_problem(run, table, compact=True):
+    """The plain path's problem; `compact`: give the supercell's shape, so
+    that the hop loop gathers from the table rows of cell 0."""
     return _native.KmcProblem(
         table=table, lattice_q=run.base_charge_config()[:, 0],
         nbr_count=run.nbr_count, nbr_site=run.nbr_site, nbr_vec=run.nbr_vec,
@@ -17,7 +19,8 @@ def native_problem(run, table):
         carrier_q=np.asarray(run.species_charge_list, dtype=np.float64),
         field=run.electric_field if run.electric_field_active else np.zeros(3),
         nu=run.material.vn, kT=run.temp, time_interval=run.time_interval,
-        n_path=int(run.t_final / run.time_interval) + 1)
+        n_path=int(run.t_final / run.time_interval) + 1,
+        cells=run.system_size if compact else None)
 
 
 def seeded_start(run, seed, n_carriers, traj=0, n_traj=1):
@@ -31,9 +34,9 @@ def seeded_start(run, seed, n_carriers, traj=0, n_traj=1):
 
 
 def run_gpu(run, table, seed, n_carriers, max_hops=None, record=True,
-            session=None, chunk=1 << 16):
+            session=None, chunk=1 << 16, compact=True):
     gen, occ = seeded_start(run, seed, n_carriers)
-    sess = session or _native.KmcSession(native_problem(run, table),
+    sess = session or _native.KmcSession(native_problem(run, table, compact),
                                          chunk=chunk)
     stream = rng.UniformStream(gen.getstate())
     res = sess.run(occ, stream, max_hops=max_hops, record=record)

@@ -46,6 +46,33 @@ def _compare(gpu, ora, n):
     return rel.max()
 
 
+def test_small_tables_keep_full_rows():
+    """Below 32 MB the whole table stays in the caches: no cell-0 rows."""
+    w = workloads.corundum(3, n_carriers=4)
+    table = w.build_table()
+    sess = _native.KmcSession(native_problem(w.make_run(table), table, True))
+    assert sess.table_rows() == w.n_sites
+    sess.close()
+
+
+@pytest.mark.parametrize("edge", [6, 8])
+def test_cell0_rows_give_the_full_table_events(edge):
+    """A periodic supercell's hop loop gathers from the table rows of cell 0
+    (the table is translation-invariant): the same events as the loop over
+    the full table, the clock within 1e-12 relative."""
+    w = workloads.corundum(edge, n_carriers=16, field_vpm=1e6)
+    table = w.build_table()
+    run = w.make_run(table)
+    for compact, rows in ((True, 12), (False, w.n_sites)):
+        sess = _native.KmcSession(native_problem(run, table, compact))
+        assert sess.table_rows() == rows
+        sess.close()
+    n = 20000
+    comp, _ = run_gpu(run, table, 2, 16, max_hops=n, compact=True)
+    full, _ = run_gpu(run, table, 2, 16, max_hops=n, compact=False)
+    print(f"edge {edge} max rel time dev", _compare(comp, full, n))
+
+
 def test_small_corundum_1e5_hops_vs_oracle():
     """3x3x3 (N = 324), 1 electron, zero field: 100000 hops, same events."""
     w = workloads.corundum(3, n_carriers=1)
