@@ -495,7 +495,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              f"trajectories are independent, see DESIGN.md for sharding",
+              f"trajectories are independent and would be sharded across "
+              f"GPUs by trajectory",
               file=sys.stderr)
         sys.exit(2)
     os.environ.setdefault("OMP_NUM_THREADS", "1")
