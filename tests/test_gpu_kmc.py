@@ -55,6 +55,22 @@ def test_small_tables_keep_full_rows():
     sess.close()
 
 
+def test_table_that_is_not_periodic_keeps_full_rows():
+    """One entry off its cell-0 translate by 1e-3 Ha: the check at creation
+    rejects the cell-0 rows and the loop reads the full table."""
+    w = workloads.corundum(6, n_carriers=4)
+    table = w.build_table().copy()
+    run = w.make_run(table)
+    table[100, 2000] += 1e-3
+    sess = _native.KmcSession(native_problem(run, table, True))
+    assert sess.table_rows() == w.n_sites
+    sess.close()
+    table[100, 2000] -= 1e-3
+    sess = _native.KmcSession(native_problem(run, table, True))
+    assert sess.table_rows() == 12
+    sess.close()
+
+
 @pytest.mark.parametrize("edge", [6, 8])
 def test_cell0_rows_give_the_full_table_events(edge):
     """A periodic supercell's hop loop gathers from the table rows of cell 0
