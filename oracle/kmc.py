@@ -69,6 +69,7 @@ def run(pb, occupancy, uniform, max_hops=None, record=False,
     energy[0] = energy_now
     time_data = [0.0]
     procs = []
+    occ_rows = [list(occ)] if record else None
     start = end = 1
     t = 0.0
     hops = 0
@@ -117,6 +118,7 @@ def run(pb, occupancy, uniform, max_hops=None, record=False,
         time_data.append(t)
         if record:
             procs.append(p)
+            occ_rows.append(list(occ))
         hops += 1
         if end >= start + 1:
             if end >= P:
@@ -133,8 +135,54 @@ def run(pb, occupancy, uniform, max_hops=None, record=False,
         "drift": drift,
         "occupancy": occ,
         "procs": np.asarray(procs, dtype=np.int64),
+        "occ_rows": (np.asarray(occ_rows, dtype=int) if record else None),
         "hops": hops,
         "start": start,
         "disp": disp,
         "energy_now": energy_now,
     }
+
+
+def drift_mobility(drift_velocity_array, field, field_mag, species_count,
+                   species_types, n_traj):
+    """`Run.compute_drift_mobility` (PyCD/core.py:2052-2082).
+
+    drift_velocity_array: (n_traj, n_carriers, 3) sums of hop vector * rate
+    (PyCD/core.py:2822-2825), atomic units.  Returns the mobility array in
+    cm^2/(V s) (n_traj, n_carriers; the drift_mobility.dat contents) and,
+    per species present, (name, mean over trajectories of the carrier mean,
+    std / sqrt(n_traj)).
+    """
+    from pycd_amd import constants
+    mob_au = np.dot(drift_velocity_array, field) / field_mag ** 2
+    mob = mob_au * (constants.BOHR2CM ** 2 * constants.SEC2AUTIME
+                    * constants.V2AUPOT)
+    per_species = []
+    start = 0
+    for s, count in enumerate(species_count):
+        end = start + count
+        if count != 0:
+            avg = np.mean(mob[:, start:end], axis=1)
+            per_species.append((species_types[s], np.mean(avg),
+                                np.std(avg) / np.sqrt(n_traj)))
+        start = end
+    return mob, per_species
+
+
+def replay_occupancy(occupancy, procs, count, nbr):
+    """Occupancy after every hop, as a doped run saves it to occupancy.npy
+    (PyCD/core.py:2765, 2772, 2818, 2905): the initial row, then one row
+    per hop.  Process p of the hop is carrier
+    c's slot j, counting the slots of every carrier in occupancy order
+    (PyCD/core.py:1957-1978)."""
+    occ = [int(s) for s in occupancy]
+    rows = [list(occ)]
+    for p in procs:
+        c = 0
+        p = int(p)
+        while p >= int(count[occ[c]]):
+            p -= int(count[occ[c]])
+            c += 1
+        occ[c] = int(nbr[occ[c], p])
+        rows.append(list(occ))
+    return np.asarray(rows, dtype=int)

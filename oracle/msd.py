@@ -6,6 +6,7 @@ trajectory means with SEM = std / sqrt(n_traj) (:3008-3036) and the D/kBT
 conversion of the least-squares slope (:3055-3071).
 """
 import numpy as np
+from scipy.stats import linregress
 
 from pycd_amd import constants
 
@@ -38,8 +39,81 @@ def brute_force_sd(track, n_lags):
     return out
 
 
+def kbt_ev(temp):
+    """kB T in eV (PyCD/core.py:2969)."""
+    return constants.KB * temp / constants.EV2J
+
+
 def diffusivity_over_kbt(slope_ang2_per_ns, n_dim, temp):
-    """slope (A^2/ns) -> D/kBT in cm^2/(V s) (PyCD/core.py:3062-3065)."""
-    kbt = constants.KB * temp / constants.EV2J
+    """slope (A^2/ns) -> D/kBT in cm^2/(V s) (PyCD/core.py:3062-3063)."""
     return (slope_ang2_per_ns * constants.ANG2CM ** 2 * constants.SEC2NS
-            / (2 * n_dim) / kbt)
+            / (2 * n_dim) / kbt_ev(temp))
+
+
+def reduce(sd_array, species_count, species_types, time_interval,
+           time_conversion, trim_length, n_dim, temp):
+    """The reduction of `compute_msd` after the lag loop
+    (PyCD/core.py:3008-3071), in the reference's order:
+
+      species_avg[t, :, s]  mean of sd over the carriers of species s
+                            (species with count 0 skipped, :3008-3027)
+      msd_data[:, 0]        lag * time_interval * time_conversion (:3030-3033)
+      msd_data[:, 1 + s]    mean over trajectories (:3034)
+      sem_data[:, s]        std over trajectories / sqrt(n_traj) (:3035-3036)
+      slope_data[t, s]      scipy.stats.linregress slope of species_avg over
+                            msd_data[:, 0], rows [trim : -trim] (:3053-3059)
+      diffusivity[s]        mean slope -> D/kBT (:3060-3063)
+      diffusivity_sem[s]    std(slopes) / sqrt(n_traj) -> D/kBT (:3066-3069)
+
+    `species_types` lists every species type of the material (count-0 ones
+    included); returns a dict of the arrays above plus the kept species.
+    """
+    sd_array = np.asarray(sd_array, dtype=np.float64)
+    n_traj, n_lags = sd_array.shape[:2]
+    counts = list(species_count)
+    n_types = len(species_types)
+    n_exist = n_types - counts.count(0)
+    species_avg = np.zeros((n_traj, n_lags, n_exist))
+    start = 0
+    skipped = 0
+    missing = []
+    for s in range(n_types):
+        if counts[s] != 0:
+            end = start + counts[s]
+            species_avg[:, :, s - skipped] = np.mean(sd_array[:, :, start:end],
+                                                     axis=2)
+            start = end
+        else:
+            skipped += 1
+            missing.append(s)
+    msd_data = np.zeros((n_lags, n_exist + 1))
+    msd_data[:, 0] = np.arange(n_lags) * time_interval * time_conversion
+    msd_data[:, 1:] = np.mean(species_avg, axis=0)
+    sem_data = np.std(species_avg, axis=0) / np.sqrt(n_traj)
+    kept = [t for i, t in enumerate(species_types) if i not in missing]
+    slope_data = np.zeros((n_traj, n_exist))
+    diff = np.zeros(n_exist)
+    diff_sem = np.zeros(n_exist)
+    x = msd_data[trim_length:-trim_length, 0]
+    for s in range(n_exist):
+        for t in range(n_traj):
+            slope_data[t, s] = linregress(
+                x, species_avg[t, trim_length:-trim_length, s])[0]
+        diff[s] = diffusivity_over_kbt(np.mean(slope_data[:, s]), n_dim, temp)
+        diff_sem[s] = diffusivity_over_kbt(
+            np.std(slope_data[:, s]) / np.sqrt(n_traj), n_dim, temp)
+    return {"species_avg": species_avg, "msd_data": msd_data,
+            "sem_data": sem_data, "slope_data": slope_data,
+            "diffusivity": diff, "diffusivity_sem": diff_sem,
+            "species_types": kept}
+
+
+def report_lines(species_types, diffusivity, diffusivity_sem):
+    """The two report lines per species (PyCD/core.py:3064-3065, 3070-3071)."""
+    out = []
+    for s, name in enumerate(species_types):
+        out.append(f"Estimated value of {name} diffusivity is: "
+                   f"{diffusivity[s]:.3e} cm2/Vs\n")
+        out.append(f"Standard error of mean in {name} diffusivity is: "
+                   f"{diffusivity_sem[s]:.3e} cm2/Vs\n")
+    return out

@@ -955,14 +955,17 @@ class Run(object):
                     self.system_relative_energies = energy
                     session.set_sites(lattice_q, energy)
                 occupancy = self.generate_initial_occupancy(dopant_sites)
-                if self.doping_active:
-                    np.save(traj_dir / "occupancy.npy",
-                            np.asarray([occupancy], dtype=int))
                 stream = rng.UniformStream(rnd.getstate())
+                # a doped run saves the occupancy after every hop, which is
+                # rebuilt from the chosen processes
                 res = session.run(occupancy, stream, ewald_neut=ewald_neut,
-                                  max_hops=max_hops, record=record)
+                                  max_hops=max_hops,
+                                  record=record or self.doping_active)
                 rnd.setstate(stream.state_after(2 * res["hops"]))
                 drift[k] = res["drift"]
+                if self.doping_active:
+                    np.save(traj_dir / "occupancy.npy",
+                            self.occupancy_rows(occupancy, res["procs"]))
                 if record:
                     self.last_records.append(res)
                 self._write_outputs(traj_dir, output_data, res, n_path)
@@ -973,6 +976,38 @@ class Run(object):
             prefix = self.compute_drift_mobility(drift, dst_path, prefix)
         generate_report(self.start_time, dst_path, "Run", 1, "".join(prefix))
         return None
+
+    def occupancy_rows(self, occupancy, procs):
+        """Occupancy before the first hop and after every hop, (hops + 1)
+        x n_c, as a doped run saves it (PyCD/core.py:2772, 2818, 2903-2906).
+        Process p is slot p % z of carrier p // z (every hopping site has
+        z slots, PyCD/core.py:1957-1978)."""
+        occ = np.asarray(occupancy, dtype=np.int64)
+        procs = np.asarray(procs, dtype=np.int64)
+        nc, hops = len(occ), len(procs)
+        z = int(self.nbr_count[occ[0]])
+        owner = procs // z
+        slot = procs - owner * z
+        nbr = self.nbr_site
+        cur = occ.tolist()
+        new_site = np.empty(hops, dtype=np.int64)
+        for h, (c, j) in enumerate(zip(owner.tolist(), slot.tolist())):
+            cur[c] = int(nbr[cur[c], j])
+            new_site[h] = cur[c]
+        rows = np.empty((hops + 1, nc), dtype=int)
+        step = np.arange(hops + 1)
+        for c in range(nc):
+            # forward fill of carrier c's site from the hops that move it
+            col = np.empty(hops + 1, dtype=np.int64)
+            col[0] = occ[c]
+            moved = np.zeros(hops + 1, dtype=bool)
+            moved[0] = True
+            hit = np.nonzero(owner == c)[0]
+            col[hit + 1] = new_site[hit]
+            moved[hit + 1] = True
+            last = np.maximum.accumulate(np.where(moved, step, 0))
+            rows[:, c] = col[last]
+        return rows
 
     def _write_outputs(self, traj_dir, output_data, res, n_path):
         arrays = {"unwrapped_traj": res["unwrapped"],

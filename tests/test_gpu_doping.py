@@ -86,7 +86,6 @@ def test_doped_do_kmc_steps_vs_oracle(hematite, tmp_path):
         with open(d / "initial_rnd_state.dump", "rb") as fh:
             random.setstate(pickle.load(fh))
         occ = run.generate_initial_occupancy(sites)
-        assert np.load(d / "occupancy.npy").tolist() == [occ]
         assert occ[0] in sites["Ti"]   # site_charge_initiation 'yes'
         pb = kmc.Problem(run.precomputed_array, q, run.nbr_count,
                          run.nbr_site, run.nbr_vec, run.nbr_lambda,
@@ -109,3 +108,8 @@ def test_doped_do_kmc_steps_vs_oracle(hematite, tmp_path):
         # the written path is the recorded one
         ut = np.load(d / "unwrapped_traj.npy")
         assert np.array_equal(ut, gpu["unwrapped"])
+        # occupancy before the first hop and after every hop, as the loop
+        # itself held it
+        occ_rows = np.load(d / "occupancy.npy")
+        assert occ_rows.shape == (n + 1, len(occ))
+        assert np.array_equal(occ_rows, ora["occ_rows"])

@@ -41,6 +41,22 @@ def test_table_from_positions(name, request):
     assert np.max(np.abs(table - ex.table)) < 1e-15 * np.max(np.abs(ex.table))
 
 
+@pytest.mark.parametrize("name", ["hematite", "bvo"])
+def test_table_rows_from_positions(name, request):
+    """`table_rows` (the large-N form: its own minimum images and a matrix
+    product over the k half-space) reproduces every row of the shipped
+    table from the shipped site order."""
+    ex = request.getfixturevalue(name)
+    al, rc, kc = ewald.ewald_units(ex.cfg["alpha"], ex.cfg["r_cut"],
+                                   ex.cfg["k_cut"])
+    n = len(ex.shipped_pos)
+    rows = ewald.table_rows(ex.shipped_pos, range(n),
+                            ex.material.lattice_matrix,
+                            np.array(ex.sim["system_size"]), ex.sim["pbc"],
+                            al, rc, kc, ex.cfg["dielectric_constant"])
+    assert np.max(np.abs(rows - ex.table)) <= 1e-15
+
+
 def test_half_space_is_a_half():
     km = np.array([2, 1, 3])
     kv = ewald.half_space_kvectors(km)
