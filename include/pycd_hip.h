@@ -155,7 +155,7 @@ typedef struct {
    * them.  With it, the hop loop gathers from the n_unit table rows of
    * cell 0 when the table is above 32 MB (the table of a periodic
    * supercell is translation-invariant; checked entry by entry at
-   * creation, to 1e-11 of the table's scale, and not used where the check
+   * creation, to 1e-14 of the table's scale, and not used where the check
    * fails). */
   int32_t cells[3];
   int32_t n_unit;
@@ -185,6 +185,21 @@ int pycd_kmc_set_sites(pycd_kmc* s, const double* lattice_q,
  * otherwise.  No reference counterpart. */
 int pycd_kmc_table_rows(const pycd_kmc* s, int64_t* rows);
 
+/* Kernel of the hop loop.  PYCD_KMC_WAVE: one wavefront that keeps the
+ * table entries of every carrier at every site the rates need in LDS
+ * (n_c <= 64, n_c z <= 256, z <= 8 and the cache within a workgroup's
+ * LDS); PYCD_KMC_WORKGROUP: a 256-thread workgroup that gathers them again
+ * each hop (any size); PYCD_KMC_AUTO (the default): the wavefront kernel
+ * where it applies.  Both give the same results bit for bit.  Asking for
+ * PYCD_KMC_WAVE where it does not apply is an argument error.
+ * pycd_kmc_variant reports the kernel the next pycd_kmc_advance runs.  No
+ * reference counterpart. */
+#define PYCD_KMC_AUTO 0
+#define PYCD_KMC_WORKGROUP 1
+#define PYCD_KMC_WAVE 2
+int pycd_kmc_set_variant(pycd_kmc* s, int32_t variant);
+int pycd_kmc_variant(const pycd_kmc* s, int32_t* variant);
+
 /* Starts a trajectory: carriers at `occupancy` (n_c sites), clock 0,
  * energy = ewald_neut + q^T A q (PyCD/core.py:2773-2780). */
 int pycd_kmc_begin(pycd_kmc* s, const int64_t* occupancy, double ewald_neut);
@@ -202,6 +217,10 @@ int pycd_kmc_advance(pycd_kmc* s, const double* uniforms, int64_t n_pairs,
                      int64_t max_hops, double* hop_time, int32_t* hop_proc,
                      int64_t* hops_out);
 
+/* Device time (ms, HIP events on the context's stream) of the kernel of
+ * the last pycd_kmc_advance call. */
+double pycd_kmc_last_ms(const pycd_kmc* s);
+
 /* Copies out the trajectory outputs (any pointer may be null):
  *   unwrapped  P x 3 n_c   (unwrapped_traj.npy)
  *   energy     P           (energy_traj.npy)
@@ -209,10 +228,6 @@ int pycd_kmc_advance(pycd_kmc* s, const double* uniforms, int64_t n_pairs,
  *   drift      n_c x 3     sum of hop vector * rate (PyCD/core.py:2822-2825)
  *   occupancy  n_c         current sites
  *   scalars    [t, energy, start_path_index, hops]                          */
-/* Device time (ms, HIP events on the context's stream) of the kernel of
- * the last pycd_kmc_advance call. */
-double pycd_kmc_last_ms(const pycd_kmc* s);
-
 int pycd_kmc_read(pycd_kmc* s, double* unwrapped, double* energy,
                   double* delg_0, double* drift, int64_t* occupancy,
                   double scalars[4]);

@@ -13,6 +13,7 @@ LIB_PATH = Path(os.environ.get(
     "PYCD_LIB", Path(__file__).resolve().parent / "lib" / "libpycd_hip.so"))
 
 PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+KMC_VARIANT = {"auto": 0, "workgroup": 1, "wave": 2}
 OK, NEED_UNIFORMS = 0, 1
 
 _lib = None
@@ -91,6 +92,8 @@ _SIGNATURES = {
     "pycd_kmc_begin": (C.c_int, [_P, _P, C.c_double]),
     "pycd_kmc_set_sites": (C.c_int, [_P, _P, _P]),
     "pycd_kmc_table_rows": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "pycd_kmc_set_variant": (C.c_int, [_P, C.c_int32]),
+    "pycd_kmc_variant": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "pycd_kmc_advance": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P,
                                    C.POINTER(C.c_int64)]),
     "pycd_kmc_read": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
@@ -214,11 +217,6 @@ def pair_table(pos, cell, trans, pbc, alpha, r_cut, k_cut, eps_r,
               "ms_real": rep.ms_real, "ms_recip": rep.ms_recip,
               "ms_total": rep.ms_total}
     return out, report
-
-
-def table_energy_parts(report, q):
-    raise NotImplementedError(
-        "compute_energy_contributions is not part of this implementation")
 
 
 class DeviceArray:
@@ -388,6 +386,20 @@ class KmcSession:
         _check(self.ctx, lib().pycd_kmc_table_rows(self.h, C.byref(n)),
                "pycd_kmc_table_rows")
         return n.value
+
+    def set_variant(self, name):
+        """Kernel of the hop loop: 'auto' (default), 'workgroup' or 'wave'
+        (the one-wavefront kernel with its LDS row cache; refused where it
+        does not apply)."""
+        _check(self.ctx, lib().pycd_kmc_set_variant(self.h, KMC_VARIANT[name]),
+               "pycd_kmc_set_variant")
+
+    def variant(self):
+        """Kernel the next advance runs: 'wave' or 'workgroup'."""
+        v = C.c_int32(0)
+        _check(self.ctx, lib().pycd_kmc_variant(self.h, C.byref(v)),
+               "pycd_kmc_variant")
+        return {1: "workgroup", 2: "wave"}[v.value]
 
     def set_sites(self, lattice_q, site_energy):
         """Lattice charges and site energies of the next trajectories."""

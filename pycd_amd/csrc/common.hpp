@@ -92,6 +92,20 @@ hipError_t download(T* dst, const T* src, size_t n, hipStream_t st) {
   return hipMemcpyAsync(dst, src, n * sizeof(T), kind, st);
 }
 
+// e**y with e = 2.718281828459045 (the double numpy's `np.e ** y` raises,
+// PyCD/core.py:2047).  That double lies below Euler's number, so
+// ln(e_double) = 1 - kLnEDefect and e_double**y = exp(y) exp(-y kLnEDefect)
+// = exp(y) (1 - y kLnEDefect) to far below an ulp for any rate exponent
+// (|y| < 1e6).  One exp and one FMA instead of an FP64 pow; measured
+// against the exactly rounded power, within 3.3e-16 relative for
+// y in [-60, 5] (the plain exp alone is off by up to 3.3e-15 there).
+constexpr double kLnEDefect = 5.318237706605891e-17;
+
+__host__ __device__ inline double numpy_e_pow(double y) {
+  const double e = exp(y);
+  return fma(-(y * kLnEDefect), e, e);
+}
+
 // Scoped device selection.
 struct DeviceGuard {
   int prev = 0;

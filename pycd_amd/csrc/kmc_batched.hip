@@ -141,7 +141,6 @@ constexpr int kPsiStride = kMaxCarriers * 8;  // n_c (1 + z), z <= 7
 constexpr int kVecStride = 3 * 3 * kMaxCarriers;
 constexpr int kMaxSubSlots = 256;  // n_unit z of a family
 constexpr int kMaxCells = 255;     // cells per axis, sites per cell
-constexpr double kEulerE = 2.718281828459045;
 constexpr double kLog2E = 1.4426950408889634;  // log2(e)
 
 // Values every lane holds identically, moved to scalar registers.
@@ -820,7 +819,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
             const double lam = e.lam;
             const double gs =
                 (((lam + g0) * (lam + g0) / (4 * lam)) - e.vab) - e.shift;
-            const double k = a.nu * pow(kEulerE, -gs / td.kT);
+            const double k = a.nu * numpy_e_pow(-gs / td.kT);
             kv[i] = real ? k : 0.0;
           }
           gv[i] = g0;
@@ -1296,11 +1295,12 @@ compact_check(const double* __restrict__ A, int64_t N, Geo g,
     const double* crow0 = A + int64_t(a >> 24) * N;
     for (int64_t j = lane; j < N; j += 64) {
       const uint32_t b = site_pack(uint32_t(j), g);
-      m = fmax(m, fabs(row[j] - crow0[rel_col(a, b, geo_pk(g))]));
+      m = nan_max(m, fabs(row[j] - crow0[rel_col(a, b, geo_pk(g))]));
     }
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  for (int off = 32; off > 0; off >>= 1)
+    m = nan_max(m, __shfl_xor(m, off, 64));
   if (lane == 0)
     atomicMax(worst, static_cast<unsigned long long>(__double_as_longlong(m)));
 }

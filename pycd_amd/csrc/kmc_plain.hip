@@ -71,7 +71,6 @@ struct KmcDev {
 };
 
 constexpr int kThreads = 256;
-constexpr double kE = 2.718281828459045;  // numpy's e
 
 // log(u2) of every uniform pair ahead of the hop loop (0 where u2 = 0, a
 // value the loop does not use), so that no hop waits for an FP64 log.
@@ -135,9 +134,11 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
     plam[p] = d.lam[sj];
     pvab[p] = d.vab[sj];
     double shift = 0.0;
-    if (d.field_on)
+    if (d.field_on) {
+#pragma clang fp contract(off)
       shift = 0.5 * (d.field[0] * d.vec[3 * sj] + d.field[1] * d.vec[3 * sj + 1] +
                      d.field[2] * d.vec[3 * sj + 2]);
+    }
     pshift[p] = shift;
     pvec[3 * p] = d.vec[3 * sj];
     pvec[3 * p + 1] = d.vec[3 * sj + 1];
@@ -228,6 +229,8 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
     __syncthreads();
     // ---- B: rates
     for (int p = tid; p < nproc; p += kThreads) {
+      // products and sums rounded one by one, as NumPy forms them
+#pragma clang fp contract(off)
       const int c = p / z;
       const int j = p - c * z;
       const double qs = d.cq[c];
@@ -238,7 +241,7 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
       const double lam = plam[p];
       const double gs = (((lam + g0) * (lam + g0) / (4 * lam)) - pvab[p]) -
                         pshift[p];
-      kk[p] = d.nu * pow(kE, -gs / d.kT);
+      kk[p] = d.nu * numpy_e_pow(-gs / d.kT);
       dg[p] = g0;
     }
     __syncthreads();
@@ -272,6 +275,7 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
         sel = nproc - 1;
       }
       if (lane == 0) {
+#pragma clang fp contract(off)
         if (!mask) ++sh_noselect;
         const int c = sel / z;
         const int s_new = pn[sel];
@@ -360,6 +364,416 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
     d.sc->start = sh_start;
     d.sc->hops = sh_hops;
     d.sc->no_select = sh_noselect;
+    if (stop) d.sc->finished = 1;
+    status[0] = code;
+    status[1] = int32_t(h);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One-wavefront instance with a row cache (used when it fits: n_c <= 64,
+// n_c z <= 256, z <= 8 and the cache within the LDS of a workgroup).
+//
+// Rows: process row p (p < n_c z) is the target site of process p, site row
+// c is carrier c's site.  The cache holds G[c][row] = A[x_row, occ_c] for
+// every carrier c and row, so psi[row] = sum_c q_c G[c][row] is summed from
+// LDS in carrier order: the FMA chain the workgroup kernel forms from its
+// gathers, so both give the same psi bit for bit.  When carrier m hops
+// s -> n only column m of the other rows changes (A[x_row, n], one gather
+// per row) and the mover's 1 + z rows are new (n_c gathers each): a hop
+// takes (1 + z)(2 n_c - 1) gathers instead of n_c (1 + z) n_c, all issued
+// together.  No gather address waits for another load: every process keeps
+// the z targets of its own target site (loaded when the process is made),
+// so the mover's new rows are in LDS when it is selected.
+//
+// Lane l owns processes l kNpl .. l kNpl + kNpl - 1 (the workgroup kernel's
+// split of the selection, so k_tot and the prefix are summed in the same
+// order) with their rate inputs in registers, and site row l (l < n_c).
+// One wavefront: no workgroup barriers; __syncthreads only waits for the
+// wavefront's own LDS operations.
+template <bool kCompact, int kNpl>
+__global__ void __launch_bounds__(64)
+kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
+                const double* __restrict__ LU, int64_t n_pairs,
+                int64_t max_hops, double* __restrict__ hop_time,
+                int32_t* __restrict__ hop_proc, int32_t* __restrict__ status) {
+  constexpr int kB = 5;  // (1 + z) n_c <= 320 mover-row gathers per lane
+  extern __shared__ double smem[];
+  const int nc = d.nc, z = d.z, mz = d.max_z, nproc = nc * z;
+  const int R = nproc + nc;
+  const int lane = threadIdx.x;
+  double* G = smem;                                   // nc x R, carrier-major
+  double* psis = G + size_t(nc) * R;                  // psi of the site rows
+  double* cqs = psis + nc;                            // carrier charges
+  double* disp = cqs + nc;                            // 3 nc
+  double* base = disp + 3 * nc;                       // 3 nc
+  double* drift = base + 3 * nc;                      // 3 nc
+  int32_t* occ = reinterpret_cast<int32_t*>(drift + 3 * nc);      // nc
+  uint32_t* opk = reinterpret_cast<uint32_t*>(occ + nc);          // nc
+  int32_t* tt = reinterpret_cast<int32_t*>(opk + nc);             // nproc z
+  uint32_t* ttpk = reinterpret_cast<uint32_t*>(tt + nproc * z);   // nproc z
+
+  // A[x, y] of the full table, or (compact) of the cell-0 row of x's
+  // sublattice at y moved by -cell(x)
+  auto entry = [&](int x, uint32_t xk, int y, uint32_t yk) -> double {
+    if constexpr (kCompact) {
+      return d.Ac[int64_t(xk >> 24) * d.N + rel_col(xk, yk, d.gp)];
+    } else {
+      return d.A[int64_t(x) * d.N + y];
+    }
+  };
+  auto row_sum = [&](int row) -> double {
+    double acc = 0.0;
+    for (int c = 0; c < nc; ++c) acc = fma(cqs[c], G[size_t(c) * R + row], acc);
+    return acc;
+  };
+
+  // ---- processes of this lane: carrier, slot, target, rate inputs
+  int pc[kNpl], pj[kNpl], px[kNpl];
+  uint32_t pxk[kNpl];
+  bool pv[kNpl];
+  double pqs[kNpl], pdphi[kNpl], ptq[kNpl], pdE[kNpl], plam[kNpl], pvab[kNpl],
+      psh[kNpl], pvx[kNpl], pvy[kNpl], pvz[kNpl], pst[kNpl];
+  // the site-dependent part of process i's rate: slot j of site s, target n
+  auto load_inputs = [&](int i, int s, int j, int n) {
+    const int64_t sj = int64_t(s) * mz + j;
+    pdphi[i] = d.phiL[n] - d.phiL[s];
+    ptq[i] = pqs[i] * d.tself[sj];
+    pdE[i] = d.E[n] - d.E[s];
+    plam[i] = d.lam[sj];
+    pvab[i] = d.vab[sj];
+    const double v0 = d.vec[3 * sj], v1 = d.vec[3 * sj + 1],
+                 v2 = d.vec[3 * sj + 2];
+    pvx[i] = v0;
+    pvy[i] = v1;
+    pvz[i] = v2;
+    double shift = 0.0;
+    if (d.field_on) {
+#pragma clang fp contract(off)
+      shift = 0.5 * (d.field[0] * v0 + d.field[1] * v1 + d.field[2] * v2);
+    }
+    psh[i] = shift;
+  };
+
+  for (int c = lane; c < nc; c += 64) {
+    const int s = d.occ[c];
+    occ[c] = s;
+    opk[c] = kCompact ? d.pk[s] : 0u;
+    cqs[c] = d.cq[c];
+  }
+  for (int k = lane; k < 3 * nc; k += 64) {
+    disp[k] = d.disp[k];
+    base[k] = d.base[k];
+    drift[k] = d.drift[k];
+  }
+  __syncthreads();
+  int last_i = -1;
+#pragma unroll
+  for (int i = 0; i < kNpl; ++i) {
+    const int p = lane * kNpl + i;
+    pv[i] = p < nproc;
+    pc[i] = pv[i] ? p / z : 0;
+    pj[i] = pv[i] ? p - pc[i] * z : 0;
+    pqs[i] = cqs[pc[i]];
+    px[i] = 0;
+    pxk[i] = 0u;
+    pst[i] = 0.0;
+    if (pv[i]) {
+      last_i = i;
+      const int s = occ[pc[i]];
+      const int n = d.nbr[int64_t(s) * mz + pj[i]];
+      px[i] = n;
+      pxk[i] = kCompact ? d.pk[n] : 0u;
+      load_inputs(i, s, pj[i], n);
+    } else {
+      pdphi[i] = ptq[i] = pdE[i] = pvab[i] = psh[i] = 0.0;
+      plam[i] = 1.0;
+      pvx[i] = pvy[i] = pvz[i] = 0.0;
+    }
+  }
+  // every process's look-ahead: the z targets of its target
+  for (int u = lane; u < nproc * z; u += 64) {
+    const int p = u / z, k = u - p * z;
+    const int c = p / z, j = p - c * z;
+    const int n = d.nbr[int64_t(occ[c]) * mz + j];
+    const int t = d.nbr[int64_t(n) * mz + k];
+    tt[u] = t;
+    ttpk[u] = kCompact ? d.pk[t] : 0u;
+  }
+  // the cache, every (carrier, row)
+  for (int u = lane; u < nc * R; u += 64) {
+    const int row = u / nc, c = u - row * nc;
+    int x;
+    uint32_t xk;
+    if (row < nproc) {
+      const int c2 = row / z, j2 = row - c2 * z;
+      x = d.nbr[int64_t(occ[c2]) * mz + j2];
+      xk = kCompact ? d.pk[x] : 0u;
+    } else {
+      x = occ[row - nproc];
+      xk = opk[row - nproc];
+    }
+    G[size_t(c) * R + row] = entry(x, xk, occ[c], opk[c]);
+  }
+  int sx = lane < nc ? occ[lane] : 0;
+  uint32_t sxk = lane < nc ? opk[lane] : 0u;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kNpl; ++i)
+    if (pv[i]) pst[i] = row_sum(lane * kNpl + i);
+  if (lane < nc) psis[lane] = row_sum(nproc + lane);
+  // the mover-row gathers this lane takes: row brl (0: the mover's site,
+  // 1 + j: its target j) at carrier bc
+  int brl[kB], bc[kB];
+#pragma unroll
+  for (int k = 0; k < kB; ++k) {
+    const int u = lane + 64 * k;
+    brl[k] = u < (1 + z) * nc ? u / nc : -1;
+    bc[k] = brl[k] >= 0 ? u - brl[k] * nc : 0;
+  }
+  // look-ahead load of this lane: target lk of the mover's new target lj
+  const int lj = lane < z * z ? lane / z : 0;
+  const int lk = lane < z * z ? lane - lj * z : 0;
+  __syncthreads();
+
+  double t = d.sc->t, energy = d.sc->energy;
+  int64_t start = d.sc->start, hops = d.sc->hops;
+  int noselect = d.sc->no_select;
+  int stop = d.sc->finished ? 1 : 0;
+  int code = stop ? 0 : -1;
+  int64_t h = 0;
+  double u1c = 0.0, u2c = 0.0, luc = 0.0;
+  while (!stop) {
+    if (max_hops > 0 && hops >= max_hops) {
+      code = 0;
+      break;
+    }
+    if (h >= n_pairs) {
+      code = 1;  // need uniforms
+      break;
+    }
+    const int hl = int(h & 63);
+    if (hl == 0) {
+      // lane l keeps the uniforms of hop h + l
+      const int64_t q = h + lane;
+      if (q < n_pairs) {
+        u1c = U[2 * q];
+        u2c = U[2 * q + 1];
+        luc = LU[q];
+      }
+    }
+    // ---- rates of this lane's processes
+    double kk[kNpl], gg[kNpl];
+    double ksum = 0.0;
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i) {
+      // products and sums rounded one by one, as NumPy forms them (and as
+      // the workgroup kernel does, whose rates pass through LDS)
+#pragma clang fp contract(off)
+      kk[i] = 0.0;
+      gg[i] = 0.0;
+      if (pv[i]) {
+        const double term01 = pdphi[i] + (pst[i] - psis[pc[i]]);
+        const double g0 = 2 * pqs[i] * (term01 + ptq[i]) + pdE[i];
+        const double lam = plam[i];
+        const double gs =
+            (((lam + g0) * (lam + g0) / (4 * lam)) - pvab[i]) - psh[i];
+        kk[i] = d.nu * numpy_e_pow(-gs / d.kT);
+        gg[i] = g0;
+        ksum += kk[i];
+      }
+    }
+    // ---- k_tot, the prefix of k / k_tot and the selection
+    const double k_tot = read_lane(wave_incl_scan(ksum), 63);
+    const double u1 = read_lane(u1c, hl);
+    const double u2 = read_lane(u2c, hl);
+    const double lu = read_lane(luc, hl);
+    double xsum = 0.0;
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i)
+      if (pv[i]) xsum += kk[i] / k_tot;
+    double cum = wave_excl_scan(xsum);
+    int found = -1;
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i) {
+      if (pv[i] && found < 0) {
+        cum += kk[i] / k_tot;
+        if (cum > u1) found = i;
+      }
+    }
+    const unsigned long long mask = __ballot(found >= 0);
+    int src;
+    if (mask) {
+      src = __ffsll(static_cast<long long>(mask)) - 1;
+    } else {
+      src = (nproc - 1) / kNpl;  // rounding left no cum > u1: the last one
+      ++noselect;
+    }
+    const int mi = found >= 0 ? found : last_i;
+    double k_my = kk[0], g_my = gg[0], vx_my = pvx[0], vy_my = pvy[0],
+           vz_my = pvz[0];
+    int c_my = pc[0], x_my = px[0];
+    uint32_t xk_my = pxk[0];
+#pragma unroll
+    for (int i = 1; i < kNpl; ++i) {
+      if (mi == i) {
+        k_my = kk[i];
+        g_my = gg[i];
+        vx_my = pvx[i];
+        vy_my = pvy[i];
+        vz_my = pvz[i];
+        c_my = pc[i];
+        x_my = px[i];
+        xk_my = pxk[i];
+      }
+    }
+    const int sel = __builtin_amdgcn_readlane(lane * kNpl + mi, src);
+    const int m = __builtin_amdgcn_readlane(c_my, src);
+    const int n = __builtin_amdgcn_readlane(x_my, src);
+    const uint32_t npk = uint32_t(__builtin_amdgcn_readlane(int(xk_my), src));
+    const double k_sel = read_lane(k_my, src);
+    const double g_sel = read_lane(g_my, src);
+    const double v_sel[3] = {read_lane(vx_my, src), read_lane(vy_my, src),
+                             read_lane(vz_my, src)};
+    // ---- clock, energy and path bookkeeping (uniform)
+    int64_t end;
+    if (u2 > 0.0) {
+      t -= lu / k_tot;
+      const double q = t / d.dt;
+      end = q < 9.0e18 ? int64_t(q) : d.P;
+    } else {
+      t = INFINITY;
+      end = d.P;
+    }
+    energy += g_sel;
+    if (lane == 0) {
+      if (hop_time) hop_time[h] = t;
+      if (hop_proc) hop_proc[h] = sel;
+    }
+    const int64_t row_lo = start;
+    int64_t row_hi = start;
+    if (end >= start + 1) row_hi = end >= d.P ? d.P : end;
+    const int64_t dp_hi = min(end, d.P);
+    ++hops;
+    // loop condition of the reference: continue while end < P
+    stop = (row_hi > row_lo && row_hi >= d.P) ? 1 : 0;
+
+    // ---- one round of loads: column m of the other rows, the mover's new
+    // rows, the rate inputs of its new processes and their look-ahead
+    const int tsel = sel * z;  // the mover's new targets: tt[tsel + j]
+    double ga[kNpl];
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i) {
+      ga[i] = 0.0;
+      if (pv[i] && pc[i] != m) ga[i] = entry(px[i], pxk[i], n, npk);
+    }
+    double gs_site = 0.0;
+    if (lane < nc && lane != m) gs_site = entry(sx, sxk, n, npk);
+    double gb[kB];
+#pragma unroll
+    for (int k = 0; k < kB; ++k) {
+      gb[k] = 0.0;
+      if (brl[k] >= 0) {
+        int x = n;
+        uint32_t xk = npk;
+        if (brl[k] > 0) {
+          x = tt[tsel + brl[k] - 1];
+          xk = ttpk[tsel + brl[k] - 1];
+        }
+        const int cc = bc[k];
+        const int y = cc == m ? n : occ[cc];
+        const uint32_t yk = cc == m ? npk : opk[cc];
+        gb[k] = entry(x, xk, y, yk);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i) {
+      if (pv[i] && pc[i] == m) {
+        const int nn = tt[tsel + pj[i]];
+        px[i] = nn;
+        pxk[i] = ttpk[tsel + pj[i]];
+        load_inputs(i, n, pj[i], nn);
+      }
+    }
+    int la = 0;
+    uint32_t lak = 0u;
+    if (lane < z * z) {
+      const int64_t at = int64_t(tt[tsel + lj]) * mz + lk;
+      la = d.nbr[at];
+      lak = kCompact ? d.nbr_pk[at] : 0u;
+    }
+    // while the loads are in flight: the mover's displacement and drift
+    if (lane < 3) {
+#pragma clang fp contract(off)
+      const double v = lane == 0 ? v_sel[0] : (lane == 1 ? v_sel[1] : v_sel[2]);
+      disp[3 * m + lane] += v;
+      if (d.field_on) drift[3 * m + lane] += v * k_sel;
+    }
+    if (lane == 0) {
+      occ[m] = n;
+      opk[m] = npk;
+    }
+    if (lane == m) {
+      sx = n;
+      sxk = npk;
+    }
+    // ---- the cache
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i)
+      if (pv[i] && pc[i] != m) G[size_t(m) * R + lane * kNpl + i] = ga[i];
+    if (lane < nc && lane != m) G[size_t(m) * R + nproc + lane] = gs_site;
+#pragma unroll
+    for (int k = 0; k < kB; ++k) {
+      if (brl[k] >= 0) {
+        const int row = brl[k] == 0 ? nproc + m : m * z + brl[k] - 1;
+        G[size_t(bc[k]) * R + row] = gb[k];
+      }
+    }
+    if (lane < z * z) {
+      tt[(m * z + lj) * z + lk] = la;
+      ttpk[(m * z + lj) * z + lk] = lak;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i)
+      if (pv[i]) pst[i] = row_sum(lane * kNpl + i);
+    if (lane < nc) psis[lane] = row_sum(nproc + lane);
+    __syncthreads();
+    // ---- path-step outputs, for a hop that ends one or more path steps
+    if (row_hi > row_lo) {
+      for (int64_t r = row_lo + lane; r < dp_hi; r += 64) d.dpath[r] = g_sel;
+      const int w = 3 * nc;
+      for (int64_t idx = lane; idx < (row_hi - row_lo) * w; idx += 64) {
+        const int64_t r = row_lo + idx / w;
+        const int k = int(idx % w);
+        d.rows[r * w + k] = base[k] + disp[k];
+      }
+      for (int64_t r = row_lo + lane; r < row_hi; r += 64) d.epath[r] = energy;
+      __syncthreads();
+      for (int k = lane; k < 3 * nc; k += 64) {
+        base[k] += disp[k];
+        disp[k] = 0.0;
+      }
+      start = row_hi;
+      __syncthreads();
+    }
+    ++h;
+    if (stop) code = 0;
+  }
+  // ---- write the state back
+  __syncthreads();
+  for (int c = lane; c < nc; c += 64) d.occ[c] = occ[c];
+  for (int k = lane; k < 3 * nc; k += 64) {
+    d.disp[k] = disp[k];
+    d.base[k] = base[k];
+    d.drift[k] = drift[k];
+  }
+  if (lane == 0) {
+    d.sc->t = t;
+    d.sc->energy = energy;
+    d.sc->start = start;
+    d.sc->hops = hops;
+    d.sc->no_select = noselect;
     if (stop) d.sc->finished = 1;
     status[0] = code;
     status[1] = int32_t(h);
@@ -455,6 +869,12 @@ struct pycd_kmc {
   DevBuf<uint32_t> pk, nbr_pk;
   double qLAqL = 0.0;
   size_t smem = 0;
+  // one-wavefront row-cache kernel: whether it fits, its LDS, processes
+  // per lane, and which kernel the hop loop runs (PYCD_KMC_*)
+  bool wave_ok = false;
+  size_t wave_smem = 0;
+  int wave_npl = 0;
+  int32_t variant = PYCD_KMC_AUTO;
   bool begun = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -514,11 +934,12 @@ periodic_gap(const double* __restrict__ A, const double* __restrict__ Ac,
     const double* row = A + i * N;
     const double* crow = Ac + int64_t(a >> 24) * N;
     for (int64_t j = lane; j < N; j += 64)
-      m = fmax(m, fabs(row[j] - crow[rel_col(key, site_pack(uint32_t(j), g),
-                                              gp)]));
+      m = nan_max(m, fabs(row[j] - crow[rel_col(key, site_pack(uint32_t(j), g),
+                                                 gp)]));
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  for (int off = 32; off > 0; off >>= 1)
+    m = nan_max(m, __shfl_xor(m, off, 64));
   if (lane == 0)
     atomicMax(worst, static_cast<unsigned long long>(__double_as_longlong(m)));
 }
@@ -527,7 +948,8 @@ periodic_gap(const double* __restrict__ A, const double* __restrict__ Ac,
 // read the n_unit rows of cell 0 (L2-resident) instead of the full table.
 // Used only when the cell shape is given, fits the packed site codes, the
 // full table is above kCellRowsFrom bytes, and every entry of the full
-// table equals its cell-0 translate to 1e-11 of the table's scale;
+// table equals its cell-0 translate to kTranslationTol of the table's
+// scale (a NaN anywhere fails);
 // otherwise the loop reads the full table.  Smaller tables stay in the
 // caches whole, and there the column arithmetic costs more than it saves
 // (768 sites: 3.51 us per hop on the full table, 3.65 on cell-0 rows;
@@ -580,7 +1002,7 @@ hipError_t compact_setup(pycd_kmc* s, const std::vector<int32_t>& nb) {
   double gap, scale = 0.0;
   std::memcpy(&gap, &bits, 8);
   for (double v : c0) scale = std::max(scale, std::fabs(v));
-  if (gap <= 1e-11 * scale) {
+  if (gap <= kTranslationTol * scale) {
     s->compact = true;
     s->geo = g;
   } else {
@@ -589,6 +1011,17 @@ hipError_t compact_setup(pycd_kmc* s, const std::vector<int32_t>& nb) {
     s->nbr_pk.release();
   }
   return hipSuccess;
+}
+
+// LDS of the one-wavefront kernel: the row cache (n_c x (n_c z + n_c)
+// doubles), site-row psi, charges, displacement/base/drift (11 n_c
+// doubles), carrier sites and their codes, and each process's look-ahead
+// (2 n_c z z words).  0 when the instance does not apply.
+size_t wave_lds_bytes(int nc, int z) {
+  const int nproc = nc * z;
+  if (nc > 64 || nproc > 256 || z > 8) return 0;
+  return sizeof(double) * (size_t(nc) * (nproc + nc) + 11 * size_t(nc)) +
+         sizeof(int32_t) * (2 * size_t(nc) + 2 * size_t(nproc) * z);
 }
 
 }  // namespace
@@ -691,8 +1124,39 @@ extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
   S_HIP(hipGetLastError());
   S_HIP(lattice_terms(s));
   S_HIP(compact_setup(s, nb));
+  {
+    int lds_max = 0;
+    S_HIP(hipDeviceGetAttribute(&lds_max,
+                                hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                ctx->device));
+    s->wave_smem = wave_lds_bytes(nc, z);
+    s->wave_npl = (nc * z + 63) / 64;
+    s->wave_ok = s->wave_smem > 0 && s->wave_smem <= size_t(lds_max);
+  }
 #undef S_HIP
   *out = s;
+  return PYCD_OK;
+}
+
+extern "C" int pycd_kmc_set_variant(pycd_kmc* s, int32_t variant) {
+  if (!s) return PYCD_ERR_ARG;
+  if (variant != PYCD_KMC_AUTO && variant != PYCD_KMC_WORKGROUP &&
+      variant != PYCD_KMC_WAVE)
+    return fail(s->ctx, PYCD_ERR_ARG, "unknown kernel variant %d", variant);
+  if (variant == PYCD_KMC_WAVE && !s->wave_ok)
+    return fail(s->ctx, PYCD_ERR_ARG,
+                "the one-wavefront kernel needs n_c <= 64, n_c z <= 256, "
+                "z <= 8 and %zu B of LDS (n_c=%d, z=%d)",
+                s->wave_smem, s->pb.n_carriers, s->pb.z);
+  s->variant = variant;
+  return PYCD_OK;
+}
+
+extern "C" int pycd_kmc_variant(const pycd_kmc* s, int32_t* variant) {
+  if (!s || !variant) return PYCD_ERR_ARG;
+  const bool wave = s->variant == PYCD_KMC_WAVE ||
+                    (s->variant == PYCD_KMC_AUTO && s->wave_ok);
+  *variant = wave ? PYCD_KMC_WAVE : PYCD_KMC_WORKGROUP;
   return PYCD_OK;
 }
 
@@ -787,6 +1251,59 @@ extern "C" int pycd_kmc_begin(pycd_kmc* s, const int64_t* occupancy,
   return PYCD_OK;
 }
 
+namespace {
+
+template <bool kCompact, int kNpl>
+hipError_t launch_wave_t(size_t smem, hipStream_t st, const KmcDev& d,
+                         const double* U, const double* LU, int64_t n_pairs,
+                         int64_t max_hops, double* T, int32_t* P,
+                         int32_t* status) {
+  if (smem > 65536) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&kmc_wave_kernel<kCompact, kNpl>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+    if (e != hipSuccess) return e;
+  }
+  kmc_wave_kernel<kCompact, kNpl>
+      <<<1, 64, smem, st>>>(d, U, LU, n_pairs, max_hops, T, P, status);
+  return hipGetLastError();
+}
+
+template <bool kCompact>
+hipError_t launch_wave_c(int npl, size_t smem, hipStream_t st,
+                         const KmcDev& d, const double* U, const double* LU,
+                         int64_t n_pairs, int64_t max_hops, double* T,
+                         int32_t* P, int32_t* status) {
+  switch (npl) {
+    case 1:
+      return launch_wave_t<kCompact, 1>(smem, st, d, U, LU, n_pairs, max_hops,
+                                        T, P, status);
+    case 2:
+      return launch_wave_t<kCompact, 2>(smem, st, d, U, LU, n_pairs, max_hops,
+                                        T, P, status);
+    case 3:
+      return launch_wave_t<kCompact, 3>(smem, st, d, U, LU, n_pairs, max_hops,
+                                        T, P, status);
+    case 4:
+      return launch_wave_t<kCompact, 4>(smem, st, d, U, LU, n_pairs, max_hops,
+                                        T, P, status);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_wave(bool compact, int npl, size_t smem, hipStream_t st,
+                       const KmcDev& d, const double* U, const double* LU,
+                       int64_t n_pairs, int64_t max_hops, double* T,
+                       int32_t* P, int32_t* status) {
+  return compact ? launch_wave_c<true>(npl, smem, st, d, U, LU, n_pairs,
+                                       max_hops, T, P, status)
+                 : launch_wave_c<false>(npl, smem, st, d, U, LU, n_pairs,
+                                        max_hops, T, P, status);
+}
+
+}  // namespace
+
 extern "C" int pycd_kmc_advance(pycd_kmc* s, const double* uniforms,
                                 int64_t n_pairs, int64_t max_hops,
                                 double* hop_time, int32_t* hop_proc,
@@ -865,12 +1382,19 @@ extern "C" int pycd_kmc_advance(pycd_kmc* s, const double* uniforms,
   if (n_pairs > 0)
     log_u2_kernel<<<int(std::min<int64_t>((n_pairs + 255) / 256, 4096)), 256,
                     0, st>>>(dU, n_pairs, s->lubuf.ptr);
-  if (s->compact)
+  const bool wave = s->variant == PYCD_KMC_WAVE ||
+                    (s->variant == PYCD_KMC_AUTO && s->wave_ok);
+  if (wave) {
+    PYCD_HIP(ctx, launch_wave(s->compact, s->wave_npl, s->wave_smem, st, d,
+                              dU, s->lubuf.ptr, n_pairs, max_hops, dT, dP,
+                              s->status.ptr));
+  } else if (s->compact) {
     kmc_plain_kernel<true><<<1, kThreads, s->smem, st>>>(
         d, dU, s->lubuf.ptr, n_pairs, max_hops, dT, dP, s->status.ptr);
-  else
+  } else {
     kmc_plain_kernel<false><<<1, kThreads, s->smem, st>>>(
         d, dU, s->lubuf.ptr, n_pairs, max_hops, dT, dP, s->status.ptr);
+  }
   PYCD_HIP(ctx, hipGetLastError());
   PYCD_HIP(ctx, hipEventRecord(s->ev1, st));
   int32_t status[2] = {-1, 0};

@@ -49,6 +49,17 @@ __host__ __device__ inline uint32_t site_add(uint32_t s, uint32_t d,
          (d & 0xff000000u);
 }
 
+// A table of a periodic supercell is used through the rows of cell 0 only
+// if every entry equals its cell-0 translate to this fraction of the
+// table's largest entry (translates built by the same arithmetic differ by
+// ~1e-16 of it).  NaN anywhere fails the check.
+constexpr double kTranslationTol = 1e-14;
+
+// max(a, b) that keeps a NaN of either side (fmax drops it).
+__host__ __device__ inline double nan_max(double a, double b) {
+  return (b != b || b > a) ? b : a;
+}
+
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 // Per-row part of a column lookup: the compact row of a's sublattice, and
