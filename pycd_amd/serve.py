@@ -22,9 +22,10 @@ Failure handling, all from the host between launches (every kernel exits
 normally): `cancel(job)` stops a pending or running job and frees its slot
 and rows; `cancel(job, requeue=True)` starts it over from its seed; a
 launch whose device time exceeds `watchdog_ms` is treated as a timed-out
-slice, and every job in it is re-queued from the start.  A job's stream
-and time grid are fixed by its parameters, so a re-run gives the same
-result.
+slice, and every job in it is re-queued from the start, up to
+`max_timeouts` times per job (then the job is rejected, so a slice that
+always overruns cannot keep the loop busy forever).  A job's stream and
+time grid are fixed by its parameters, so a re-run gives the same result.
 """
 import ctypes as C
 import time
@@ -90,6 +91,7 @@ class Job:
         self.slot = None
         self.rows_at = None
         self.n_path = None
+        self.timeouts = 0
 
 
 class KmcServer:
@@ -97,9 +99,10 @@ class KmcServer:
                  rows_capacity=1 << 28, table_budget_bytes=32 << 30,
                  slice_hops=2000, rate_precision="f32", path_rows=200,
                  msd_lags=100, waves_per_cu=8, nu=None, watchdog_ms=None,
-                 sub_max=64):
+                 sub_max=64, max_timeouts=2):
         """family_builder(edge) -> (Run, table) for that lattice size.
-        sub_max bounds sites-per-cell x hop slots of the families served."""
+        sub_max bounds sites-per-cell x hop slots of the families served;
+        a job in more than `max_timeouts` timed-out slices is rejected."""
         self.build_family = family_builder
         self.slice_hops = int(slice_hops)
         self.path_rows = int(path_rows)
@@ -110,7 +113,7 @@ class KmcServer:
         self.msd_lags = int(msd_lags)
         self.table_budget = int(table_budget_bytes)
         self.nc_max = int(nc_max)
-        self.ctx = _native.context()
+        self.ctx = None             # the device context, with the batch
         self.families = {}          # edge -> (index, Family)
         self.lru = OrderedDict()    # edge -> last use
         self.family_bytes = {}
@@ -130,13 +133,19 @@ class KmcServer:
         self.launch_ms = []
         self.evictions = 0
         self.watchdog_ms = watchdog_ms
+        self.max_timeouts = int(max_timeouts)
         self.timeouts = 0
         self.requeued = 0
+        # device footprint per edge, and the last family built only to
+        # measure it (taken by the admission that follows, not built again)
+        self._footprint = {}
+        self._staged = None
 
     # -- device objects ----------------------------------------------------
     def _ensure_batch(self, nu):
         if self.h is not None:
             return
+        self.ctx = _native.context()
         h = C.c_void_p()
         _check(self.ctx, lib().pycd_batch_create(
             self.ctx, None, 0, float(nu), _native.PREC["f64"],
@@ -168,8 +177,11 @@ class KmcServer:
         if edge in self.families:
             self.lru.move_to_end(edge)
             return self.families[edge][0]
-        run, table = self.build_family(edge)
-        fam = Family(run, table)
+        if self._staged is not None and self._staged[0] == edge:
+            fam = self._staged[1]
+        else:
+            fam = Family(*self.build_family(edge))
+        self._staged = (edge, fam)
         need = fam.device_bytes("f64")
         busy = self._busy_edges()
         while sum(self.family_bytes.values()) + need > self.table_budget:
@@ -183,6 +195,7 @@ class KmcServer:
                                                      C.byref(idx)),
                "pycd_batch_add_family")
         fam.table = None  # the device holds it now
+        self._staged = None
         self.families[edge] = (idx.value, fam)
         self.family_bytes[edge] = need
         self.lru[edge] = time.perf_counter()
@@ -219,9 +232,16 @@ class KmcServer:
         return None
 
     def table_bytes(self, edge):
-        """Device bytes of the family's compact form (cell-0 table rows)."""
-        run, table = self.build_family(edge)
-        return Family(run, table).device_bytes("f64")
+        """Device bytes of the family's compact form (cell-0 table rows),
+        measured once per edge."""
+        if edge not in self._footprint:
+            if self._staged is not None and self._staged[0] == edge:
+                fam = self._staged[1]
+            else:
+                fam = Family(*self.build_family(edge))
+                self._staged = (edge, fam)
+            self._footprint[edge] = fam.device_bytes("f64")
+        return self._footprint[edge]
 
     def _try_admit(self, job):
         if not self.free_slots:
@@ -299,8 +319,7 @@ class KmcServer:
         elif job.slot is not None and self.active.get(job.slot) is job:
             s = job.slot
             del self.active[s]
-            _check(self.ctx, lib().pycd_batch_release(self.h, s),
-                   "pycd_batch_release")
+            self._release_slot(s)
             self.rows.release(job.rows_at, job.n_path * 3 * job.n_carriers)
             self.free_slots.append(s)
         else:
@@ -315,22 +334,38 @@ class KmcServer:
             self.rejected.append(job)
         return True
 
-    def step(self):
-        """One slice over every admitted job; retire the finished ones."""
-        if not self.active:
-            return []
+    def _release_slot(self, slot):
+        _check(self.ctx, lib().pycd_batch_release(self.h, slot),
+               "pycd_batch_release")
+
+    def _launch_slice(self):
+        """One pycd_batch_step launch; returns its device time in ms."""
         ms = C.c_double(0.0)
         _check(self.ctx, lib().pycd_batch_step(self.h, self.slice_hops,
                                                C.byref(ms)),
                "pycd_batch_step")
-        self.launch_ms.append(ms.value)
-        if self.watchdog_ms is not None and ms.value > self.watchdog_ms:
+        return ms.value
+
+    def step(self):
+        """One slice over every admitted job; retire the finished ones."""
+        if not self.active:
+            return []
+        ms = self._launch_slice()
+        self.launch_ms.append(ms)
+        if self.watchdog_ms is not None and ms > self.watchdog_ms:
             # a timed-out slice: its jobs start over, newest first so that
-            # the queue keeps their arrival order
+            # the queue keeps their arrival order; a job past its limit of
+            # timed-out slices is rejected instead
             self.timeouts += 1
             for job in sorted(self.active.values(),
                               key=lambda j: -j.arrival):
-                self.cancel(job, requeue=True)
+                job.timeouts += 1
+                if job.timeouts > self.max_timeouts:
+                    self.cancel(job)
+                    job.result = {"error": f"timed out in {job.timeouts} "
+                                           f"slices", "hops": 0}
+                else:
+                    self.cancel(job, requeue=True)
             return []
         res = (TrajResultC * self.n_slots)()
         _check(self.ctx, lib().pycd_batch_results(self.h, res, None),
@@ -359,8 +394,7 @@ class KmcServer:
                           "rows": rows,
                           "msd": msd[k * self.msd_lags:(k + 1) * self.msd_lags]}
             job.finished = now
-            _check(self.ctx, lib().pycd_batch_release(self.h, s),
-                   "pycd_batch_release")
+            self._release_slot(s)
             self.rows.release(job.rows_at, job.n_path * 3 * job.n_carriers)
             self.free_slots.append(s)
             self.done.append(job)

@@ -86,3 +86,75 @@ def test_bench_refuses_more_than_one_gpu(capsys, monkeypatch):
     else:
         raise AssertionError("bench.py accepted --gpus 2")
     assert "one GPU" in capsys.readouterr().err
+
+
+class _StalledServer:
+    """KmcServer whose every slice overruns the watchdog; admission and slot
+    release are stubbed, so no device is touched."""
+
+    @staticmethod
+    def make(**kw):
+        from pycd_amd.serve import KmcServer
+
+        class Stalled(KmcServer):
+            launches = 0
+
+            def _try_admit(self, job):
+                if not self.free_slots:
+                    return False
+                s = self.free_slots.popleft()
+                job.slot, job.rows_at, job.n_path = s, 0, 1
+                self.active[s] = job
+                return True
+
+            def _never_fits(self, job):
+                return None
+
+            def _release_slot(self, slot):
+                pass
+
+            def _launch_slice(self):
+                Stalled.launches += 1
+                return 1e9
+
+        srv = Stalled(lambda edge: None, n_slots=4, rows_capacity=1 << 10,
+                      watchdog_ms=10.0, **kw)
+        srv.rows.release = lambda lo, n: None
+        return srv
+
+
+def test_watchdog_requeue_is_bounded():
+    """Slices that always overrun the watchdog: each job is re-queued
+    max_timeouts times and then rejected, so drain() ends."""
+    from pycd_amd.serve import Job
+    srv = _StalledServer.make(max_timeouts=2)
+    jobs = [Job(k, edge=3, n_carriers=1, hops=10, seed=k) for k in range(6)]
+    for j in jobs:
+        srv.submit(j)
+    srv.drain()
+    assert all(j.result["error"] == "timed out in 3 slices" for j in jobs)
+    assert srv.requeued == 2 * len(jobs)
+    assert not srv.active and not srv.pending
+    srv.h = None
+
+
+def test_footprint_is_measured_once_per_edge(monkeypatch):
+    """The table footprint of an edge is computed from one family build,
+    which the following admission reuses instead of building again."""
+    from pycd_amd import serve
+    calls = []
+
+    class FakeFamily:
+        def __init__(self, run, table):
+            calls.append(run)
+
+        def device_bytes(self, precision):
+            return 1234
+
+    monkeypatch.setattr(serve, "Family", FakeFamily)
+    srv = serve.KmcServer(lambda edge: (edge, None), n_slots=2)
+    assert srv.table_bytes(5) == 1234
+    assert srv.table_bytes(5) == 1234
+    assert calls == [5]
+    assert srv._staged[0] == 5
+    srv.h = None
