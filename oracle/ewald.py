@@ -181,3 +181,23 @@ def table_rows(pos, rows, lattice_bohr, system_size, pbc, alpha, r_cut,
         row[i] += -np.sqrt(alpha / np.pi) / eps_r
         out[r_i] = row
     return out
+
+
+def k_vector_energies(pairwise, recip, volume, k_max, alpha, k_cut, q):
+    """Energy of charges q per half-space k vector, as
+    `pot_k_ewald_with_k_vector_data` forms it (PyCD/core.py:880-911): the
+    sum over q_i q_j of that k vector's term of the table, before the
+    factor 2; 0 for |k|^2 >= k_cut^2.  Returns (k triples, energies)."""
+    kvs = half_space_kvectors(k_max)
+    qq = np.outer(q, q)
+    alpha4 = 4 * alpha
+    coeff = (2 * np.pi) / volume
+    out = np.zeros(len(kvs))
+    for idx, kv in enumerate(kvs):
+        k = np.dot(kv, recip)
+        k2 = np.dot(k, k)
+        if k2 < k_cut ** 2:
+            term = (coeff * np.exp(-k2 / alpha4)
+                    * np.cos(np.tensordot(pairwise, k, axes=([2], [0]))) / k2)
+            out[idx] = np.sum(np.multiply(qq, term))
+    return kvs, out

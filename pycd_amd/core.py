@@ -394,12 +394,14 @@ class System(object):
         """Ewald pair table on the GPU (PyCD/core.py:1611-1679).
 
         Returns (A, dst_path) with A the N x N FP64 table, and writes
-        `precomputed_array.log` with alpha on its fourth line.
+        `precomputed_array.log` with alpha on its fourth line.  With
+        `compute_energy_contributions` the log also gets the real-space,
+        Fourier-space and self-interaction energies of the lattice charges
+        (PyCD/core.py:1662-1675); with `return_k_vector_data` the energy of
+        each half-space k vector goes, largest first, to
+        `k_vector_energy_contribution_within_user_specified_k_cut.log`
+        (PyCD/core.py:1628-1649).
         """
-        if return_k_vector_data:
-            raise NotImplementedError(
-                "per-k-vector energy analysis is not part of this "
-                "implementation")
         alpha, r_cut, k_cut = self._ewald_parameters()
         pos = self.neighbors.bulk_sites.cell_coordinates
         table, report = _native.pair_table(
@@ -437,8 +439,10 @@ class System(object):
         n_est = np.ceil(np.prod(2 * k_max + 1) * np.pi / 6 - 1).astype(int)
         lines.append(f"k_max: [{k_max[0]}, {k_max[1]}, {k_max[2]}]\n")
         lines.append(f"number of k-vectors: {n_est}\n\n")
+        if return_k_vector_data:
+            self._k_vector_report(q[:, 0], alpha, k_cut, k_max, dst_path)
         if compute_energy_contributions:
-            parts = _native.table_energy_parts(report, q[:, 0])
+            parts = self.energy_parts(table, q[:, 0], alpha, r_cut)
             ev = constants.EV2HARTREE
             lines.append(f"Energy contribution from Real space: "
                          f"{parts['real'] / ev} eV\n")
@@ -452,6 +456,77 @@ class System(object):
             generate_report(self.start_time, Path(dst_path),
                             "precomputed_array", 1, "".join(lines))
         return (table, dst_path)
+
+    def energy_parts(self, table, q, alpha, r_cut):
+        """q^T A q split into its real-space, Fourier-space and self parts
+        (PyCD/core.py:1662-1675) for charges q.  The real and self parts
+        come from a second table without reciprocal sum (k_cut = 0) on the
+        GPU; the self part is -sqrt(alpha / pi) / eps sum q^2."""
+        eps = self.material.dielectric_constant
+        pos = self.neighbors.bulk_sites.cell_coordinates
+        real_self, _ = _native.pair_table(
+            pos, self.neighbors.simulation_cell_matrix,
+            self.translational_matrix, self.pbc, alpha, r_cut, 0.0, eps,
+            "f64")
+        q = np.asarray(q, dtype=np.float64)
+        e_self = -np.sqrt(alpha / np.pi) / eps * float(q @ q)
+        e_rs = float(q @ real_self @ q)
+        e_all = float(q @ table @ q)
+        real, fourier = e_rs - e_self, e_all - e_rs
+        return {"real": real, "fourier": fourier, "self": e_self,
+                "total": real + fourier + e_self}
+
+    @staticmethod
+    def half_space(k_max):
+        """Integer k triples of [-k_max, k_max]^3 without 0, one of each
+        +/- pair, in the order of get_effective_k_vectors
+        (PyCD/core.py:816-838): mgrid order, keeping v unless -v came
+        earlier, i.e. v whose first non-zero component is negative."""
+        g = np.mgrid[-k_max[0]:k_max[0] + 1, -k_max[1]:k_max[1] + 1,
+                     -k_max[2]:k_max[2] + 1].reshape(3, -1).T
+        a, b, c = g[:, 0], g[:, 1], g[:, 2]
+        keep = (a < 0) | ((a == 0) & (b < 0)) | ((a == 0) & (b == 0) & (c < 0))
+        return g[keep]
+
+    def k_vector_energies(self, q, alpha, k_cut, k_max):
+        """Energy of charges q per half-space k vector (PyCD/core.py:880-911,
+        before the factor 2 and the dielectric constant):
+        (2 pi / V) e^{-k^2/4 alpha} / k^2 sum_ij q_i q_j cos(k . r_ij) for
+        |k|^2 < k_cut^2, 0 beyond.  k is a reciprocal vector of the
+        supercell, so cos(k . r_ij) is the same for any image of r_ij and
+        the double sum is |S(k)|^2 with S(k) = sum_i q_i e^{i k . r_i}."""
+        kv = self.half_space(k_max)
+        k = kv @ self.reciprocal_lattice_matrix
+        k2 = np.einsum("ij,ij->i", k, k)
+        inside = k2 < k_cut ** 2
+        pos = self.neighbors.bulk_sites.cell_coordinates
+        q = np.asarray(q, dtype=np.float64)
+        out = np.zeros(len(kv))
+        idx = np.nonzero(inside)[0]
+        for lo in range(0, len(idx), 256):
+            sel = idx[lo:lo + 256]
+            phase = pos @ k[sel].T
+            s2 = (q @ np.cos(phase)) ** 2 + (q @ np.sin(phase)) ** 2
+            out[sel] = ((2 * np.pi) / self.system_volume
+                        * np.exp(-k2[sel] / (4 * alpha)) * s2 / k2[sel])
+        return kv, out
+
+    def _k_vector_report(self, q, alpha, k_cut, k_max, dst_path):
+        kv, energy = self.k_vector_energies(q, alpha, k_cut, k_max)
+        order = np.argsort(energy)[::-1]
+        lines = ["k-vectors sorted in the decreasing order of their energy "
+                 "contributions\n"]
+        for i in order:
+            v = kv[i]
+            lines.append(f"{v[0]:4d} {v[1]:4d} {v[2]:4d}: "
+                         f"{energy[i] / constants.EV2HARTREE:.3e} eV\n")
+        if dst_path is not None:
+            generate_report(
+                self.start_time, Path(dst_path),
+                "k_vector_energy_contribution_within_user_specified_k_cut", 0,
+                "".join(lines))
+        self.k_vector_data = kv
+        self.k_vector_energy = energy
 
 
 class Run(object):

@@ -73,3 +73,56 @@ def test_system_api_writes_log(tmp_path, hematite):
                            np.array(hematite.sim["system_size"]), al, rc, kc,
                            hematite.cfg["dielectric_constant"])
     assert np.max(np.abs(table - ref)) / np.max(np.abs(ref)) <= REL_TOL
+
+
+@pytest.mark.parametrize("name", ["hematite", "bvo"])
+def test_energy_contributions_and_k_vector_report(tmp_path, name, request):
+    """get_precomputed_array(dst, 1, 1): the real-space, Fourier-space and
+    self energies of the lattice charges in the log equal q^T A q of the
+    oracle's separate parts (PyCD/core.py:1662-1675), and the k-vector
+    report lists every half-space vector with the oracle's energy, largest
+    first (PyCD/core.py:1628-1649)."""
+    import re
+    from pycd_amd import constants
+    ex = request.getfixturevalue(name)
+    system = ex.system
+    table, _ = system.get_precomputed_array(tmp_path, 1, 1)
+    al, rc, kc = ewald.ewald_units(ex.cfg["alpha"], ex.cfg["r_cut"],
+                                   ex.cfg["k_cut"])
+    eps = ex.cfg["dielectric_constant"]
+    lat = ex.material.lattice_matrix
+    size = np.array(ex.sim["system_size"])
+    pos = ex.neighbors.bulk_sites.cell_coordinates
+    pw = ewald.pairwise_vectors(pos, ex.neighbors.simulation_cell_matrix,
+                                ex.sim["pbc"])
+    _, volume, recip = ewald.geometry(lat, size)
+    km = ewald.k_max_for(kc, recip)
+    q = system.base_charge_config_for_accuracy_analysis("full")[:, 0]
+    real = q @ (ewald.pot_r(pw, al, rc)[0] / eps) @ q
+    fourier = q @ (ewald.pot_k(pw, recip, volume, km, al, kc) / eps) @ q
+    self_e = -np.sqrt(al / np.pi) / eps * (q @ q)
+    text = (tmp_path / "precomputed_array.log").read_text()
+    got = {k: float(v) for k, v in re.findall(
+        r"Energy contribution from (Real space|Fourier-space|"
+        r"self-interactions): ([-+0-9.e]+) eV", text)}
+    total = float(re.search(r"Total system energy \(neutral\): "
+                            r"([-+0-9.e]+) eV", text).group(1))
+    ev = constants.EV2HARTREE
+    scale = max(abs(real), abs(fourier), abs(self_e))
+    for key, ref in (("Real space", real), ("Fourier-space", fourier),
+                     ("self-interactions", self_e)):
+        assert abs(got[key] * ev - ref) <= 1e-11 * scale, key
+    assert abs(total * ev - (real + fourier + self_e)) <= 1e-11 * scale
+    kv, e = ewald.k_vector_energies(pw, recip, volume, km, al, kc, q)
+    rep = (tmp_path / "k_vector_energy_contribution_within_user_specified_"
+           "k_cut.log").read_text().splitlines()
+    assert rep[0].startswith("k-vectors sorted in the decreasing order")
+    rows = [r for r in rep[1:] if re.match(r"^\s*-?\d+\s+-?\d+\s+-?\d+:", r)]
+    assert len(rows) == len(kv)
+    listed = [float(r.split(":")[1].split()[0]) for r in rows]
+    assert listed == sorted(listed, reverse=True)
+    top = np.argmax(e)
+    a, b, c = (int(x) for x in rows[0].split(":")[0].split())
+    assert abs(e[top] - e[[i for i, v in enumerate(kv)
+                           if tuple(v) == (a, b, c)][0]]) <= 1e-9 * e[top]
+    assert listed[0] == pytest.approx(e[top] / ev, rel=2e-3)

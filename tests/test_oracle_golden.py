@@ -244,3 +244,35 @@ def test_msd_oracle_ballistic():
     sd = msd.squared_displacement(pos, 1, P, M)
     tau = np.arange(M)
     assert np.allclose(sd[0, :, 0], (v @ v) * tau ** 2, rtol=1e-13)
+
+
+def test_half_space_order_matches_product():
+    from pycd_amd.core import System
+    for km in ([2, 1, 3], [0, 0, 2], [1, 2, 0], [3, 3, 3]):
+        assert np.array_equal(System.half_space(np.array(km)),
+                              ewald.half_space_kvectors(np.array(km)))
+
+
+@pytest.mark.parametrize("name", ["hematite", "bvo"])
+def test_k_vector_energies(name, request):
+    """Per-k energies of the lattice charges: the product's structure-factor
+    form |S(k)|^2 equals the reference's double sum over the shipped
+    minimum-image vectors, and twice their sum is q^T A_k q (the
+    reciprocal part of the table)."""
+    ex = request.getfixturevalue(name)
+    al, rc, kc = ewald.ewald_units(ex.cfg["alpha"], ex.cfg["r_cut"],
+                                   ex.cfg["k_cut"])
+    lat = ex.material.lattice_matrix
+    size = np.array(ex.sim["system_size"])
+    _, volume, recip = ewald.geometry(lat, size)
+    km = ewald.k_max_for(kc, recip)
+    system = ex.system
+    q = system.base_charge_config_for_accuracy_analysis("full")[:, 0]
+    kv_o, e_o = ewald.k_vector_energies(ex.pairwise, recip, volume, km, al,
+                                        kc, q)
+    # the product's own geometry, from the site order it built
+    kv_p, e_p = system.k_vector_energies(q, al, kc, km)
+    assert np.array_equal(kv_p, kv_o)
+    assert np.allclose(e_p, e_o, rtol=1e-9, atol=1e-12 * np.max(np.abs(e_o)))
+    full = ewald.pot_k(ex.pairwise, recip, volume, km, al, kc)
+    assert 2 * e_o.sum() == pytest.approx(q @ full @ q, rel=1e-12)
