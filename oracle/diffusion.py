@@ -20,9 +20,32 @@ import numpy as np
 from pycd_amd import constants
 
 
-def exact_diffusivity(nbr, vec, lam, vab, count, sublattice, nu, kT):
+def exact_diffusivity(nbr, vec, lam, vab, count, sublattice, nu, kT,
+                      field=None):
     """All inputs atomic units; returns (D tensor in bohr^2/au_time,
-    D_scalar in cm^2/s)."""
+    D_scalar in cm^2/s).  See `markov_additive` for the field."""
+    _, D, cm2 = markov_additive(nbr, vec, lam, vab, count, sublattice, nu, kT,
+                                field)
+    return D, cm2
+
+
+def exact_drift_velocity(nbr, vec, lam, vab, count, sublattice, nu, kT,
+                         field):
+    """Mean drift velocity (bohr per au time) of one carrier in a constant
+    field: with one charge dG0 is still zero on equivalent sites and each
+    slot's rate is nu exp(-(lambda/4 - V_ab - 0.5 E.d)/kT) (the field term of
+    PyCD/core.py:2041-2044, independent of the carrier's sign), so the
+    displacement grows at v = sum_a pi_a m_a."""
+    v, _, _ = markov_additive(nbr, vec, lam, vab, count, sublattice, nu, kT,
+                              field)
+    return v
+
+
+def markov_additive(nbr, vec, lam, vab, count, sublattice, nu, kT,
+                    field=None):
+    """(v, D tensor, D_scalar in cm^2/s) of the sublattice walk; `field`
+    (au, default zero) shifts each slot's barrier by -0.5 E.d."""
+    field = np.zeros(3) if field is None else np.asarray(field, float)
     subs = np.unique(sublattice[count > 0])
     index = {int(a): i for i, a in enumerate(subs)}
     na = len(subs)
@@ -33,7 +56,8 @@ def exact_diffusivity(nbr, vec, lam, vab, count, sublattice, nu, kT):
         s = int(np.nonzero((sublattice == a) & (count > 0))[0][0])
         ia = index[int(a)]
         for j in range(int(count[s])):
-            k = nu * np.exp(-(lam[s, j] / 4 - vab[s, j]) / kT)
+            shift = 0.5 * float(np.dot(field, vec[s, j]))
+            k = nu * np.exp(-((lam[s, j] / 4 - vab[s, j]) - shift) / kT)
             ib = index[int(sublattice[nbr[s, j]])]
             Q[ia, ib] += k
             Q[ia, ia] -= k
@@ -53,7 +77,7 @@ def exact_diffusivity(nbr, vec, lam, vab, count, sublattice, nu, kT):
     D = two_d / 2
     d_scalar = np.trace(D) / 3
     cm2_per_s = d_scalar * (constants.BOHR2CM ** 2) * constants.SEC2AUTIME
-    return D, cm2_per_s
+    return v, D, cm2_per_s
 
 
 def over_kbt(d_cm2_per_s, temp):

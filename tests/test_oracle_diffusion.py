@@ -1,5 +1,6 @@
 """The exact-diffusivity oracle against closed forms."""
 import numpy as np
+import pytest
 
 from oracle.diffusion import exact_diffusivity, over_kbt
 from oracle.neighbors import hop_neighbor_list
@@ -40,3 +41,29 @@ def test_corundum_value():
                                run.material.vn, run.temp)
     assert abs(dcm / 1.136162e-3 - 1) < 2e-6
     assert abs(over_kbt(dcm, 300.0) / 4.394872e-2 - 1) < 2e-6
+
+
+def test_drift_velocity_one_sublattice_closed_form():
+    """Simple cubic lattice, field along x: v_x = a (k+ - k-) with
+    k+- = nu exp(-(lambda/4 - V_ab -+ 0.5 E a)/kT), and no drift across the
+    field; at zero field no drift at all."""
+    from oracle.diffusion import exact_drift_velocity
+    nbr = np.array([[1, 1, 1, 1, 1, 1], [0, 0, 0, 0, 0, 0]])
+    a = 3.0
+    d = np.array([[a, 0, 0], [-a, 0, 0], [0, a, 0], [0, -a, 0], [0, 0, a],
+                  [0, 0, -a]], dtype=float)
+    vec = np.stack([d, d])
+    lam = np.full((2, 6), 0.03)
+    vab = np.full((2, 6), 0.002)
+    count = np.array([6, 6])
+    sub = np.array([0, 0])
+    nu, kT, E = 1e-3, 0.001, 2e-4
+    v = exact_drift_velocity(nbr, vec, lam, vab, count, sub, nu, kT,
+                             [E, 0.0, 0.0])
+    kp = nu * np.exp(-((0.03 / 4 - 0.002) - 0.5 * E * a) / kT)
+    km = nu * np.exp(-((0.03 / 4 - 0.002) + 0.5 * E * a) / kT)
+    assert v[0] == pytest.approx(a * (kp - km), rel=1e-12)
+    assert abs(v[1]) < 1e-12 * abs(v[0]) and abs(v[2]) < 1e-12 * abs(v[0])
+    v0 = exact_drift_velocity(nbr, vec, lam, vab, count, sub, nu, kT,
+                              [0.0, 0.0, 0.0])
+    assert np.all(np.abs(v0) < 1e-15)
