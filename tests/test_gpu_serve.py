@@ -223,3 +223,66 @@ def test_served_jobs_equal_the_batched_trajectories():
             (res["t"][k], res["energy"][k], res["hops"][k], res["start"][k]), k
         assert np.array_equal(r["rows"], bt.rows(k)), k
     bt.close()
+
+
+_MONO = {}
+
+
+def monoclinic_family(edge):
+    if edge not in _MONO:
+        w = workloads.monoclinic(edge)
+        table = w.build_table()
+        _MONO[edge] = (w.make_run(table), table)
+    return _MONO[edge]
+
+
+def test_served_monoclinic_jobs_vs_oracle():
+    """The serving form's own lattice (monoclinic, 4 sites per cell, hops to
+    2.886 and 2.902 A) through the server in 257-hop slices with FP64
+    rates: every job's hop count, clock, path rows and energy equal the
+    oracle's trajectory on the same seed and time grid
+    (PyCD/core.py:2647-2861)."""
+    import random
+    from oracle import kmc, seeding
+    from pycd_amd import constants
+    from tests.systems import oracle_problem
+    rng = np.random.default_rng(21)
+    specs = [dict(edge=int(rng.integers(4, 7)),
+                  n_carriers=int(rng.integers(1, 9)),
+                  hops=int(rng.integers(600, 1500)), seed=700 + k,
+                  temp=float(rng.uniform(260, 380)),
+                  field_vpm=float(rng.choice([0.0, 3e6])))
+             for k in range(8)]
+    srv = KmcServer(monoclinic_family, n_slots=16, nc_max=8, z_max=2,
+                    rows_capacity=1 << 22, slice_hops=257,
+                    rate_precision="f64", path_rows=40, msd_lags=10)
+    jobs = [Job(k, **s) for k, s in enumerate(specs)]
+    for j in jobs:
+        srv.submit(j)
+    srv.drain()
+    n_path = srv.n_path
+    srv.close()
+    assert len({s["edge"] for s in specs}) > 1
+    for j, s in zip(jobs, specs):
+        run, table = monoclinic_family(s["edge"])
+        nc = s["n_carriers"]
+        field = (np.array([0.0, 0.0, 1.0]) * s["field_vpm"]
+                 * constants.VPM2AUFIELD if s["field_vpm"] else np.zeros(3))
+        pb = oracle_problem(run, table=table, field=field)
+        pb.kT = s["temp"] * constants.K2AUTEMP
+        pb.carrier_q = np.full(nc, -1.0)
+        pb.time_interval = j.time_interval
+        pb.n_path = n_path
+        gen = random.Random()
+        gen.setstate(seeding.trajectory_states(s["seed"], 1)[0])
+        occ = seeding.initial_occupancy(gen, run.hop_site_lists["electron"],
+                                        nc)
+        ora = kmc.run(pb, occ, gen.random, max_hops=s["hops"])
+        r = j.result
+        assert r["hops"] == ora["hops"] > 0, j.id
+        assert abs(r["t"] - ora["time_data"][-1]) <= \
+            1e-10 * ora["time_data"][-1], j.id
+        assert r["start"] == ora["start"], j.id
+        assert np.max(np.abs(r["rows"] - ora["unwrapped"])) < 1e-8, j.id
+        scale = max(1.0, abs(ora["energy_now"]))
+        assert abs(r["energy"] - ora["energy_now"]) < 1e-9 * scale, j.id
