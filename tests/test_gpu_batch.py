@@ -57,7 +57,21 @@ def test_fp64_batch_matches_oracle_per_trajectory():
     fams, runs = _families([3, 4], t_final=2e-8)
     specs = [(0, 1, 0.0, 300.0), (1, 5, 1e6, 300.0), (0, 16, 5e6, 350.0),
              (1, 3, 0.0, 250.0), (1, 16, 2e6, 400.0), (0, 2, 1e7, 280.0)]
-    seed, max_hops = 7, 1500
+    _fp64_vs_oracle(fams, runs, specs, 7, 1500)
+
+
+def test_fp64_batch_matches_oracle_many_carriers():
+    """The batched form's large end: 32 and 64 electrons on the 6x6x6
+    (2592 sites) and 8x8x8 (6144 sites) families, fields up to 1e7 V/m,
+    250-400 K; per trajectory the oracle's hops, clock, rows, energy and
+    drift (the kernel's large instance runs the 64-electron ones)."""
+    fams, runs = _families([6, 8], t_final=2e-8)
+    specs = [(0, 32, 1e6, 300.0), (1, 64, 5e6, 350.0), (1, 32, 0.0, 250.0),
+             (0, 64, 1e7, 400.0)]
+    _fp64_vs_oracle(fams, runs, specs, 13, 800)
+
+
+def _fp64_vs_oracle(fams, runs, specs, seed, max_hops):
     trajs, occs, words = _mixed_batch(fams, runs, specs, seed, max_hops)
     bt = B.KmcBatch(fams, "f64", "f64")
     order = np.argsort([-s[1] for s in specs], kind="stable")

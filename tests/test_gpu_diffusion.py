@@ -125,3 +125,53 @@ def test_fp32_table_batch_diffusivity_within_two_sigma():
     exact = over_kbt(dcm, 300.0)
     print(f"GPU FP32-table D/kBT = {mean:.5e} +- {sem:.2e}; exact {exact:.5e}")
     assert abs(mean - exact) <= 2 * sem
+
+
+@pytest.mark.parametrize("temp,direction,seed", [
+    (250.0, (1, 0, 0), 41),   # along a: fast basal hops, drift ~100 SEM
+    (400.0, (0, 0, 1), 43),   # along c: slow c hops, drift ~10 SEM
+])
+def test_fp32_batch_drift_velocity_within_two_sigma(temp, direction, seed):
+    """One electron, |E| = 1e7 V/m along a lattice direction, FP32 rates:
+    the mean velocity along the field of 256 trajectories of 20000 hops
+    (path rows, start to last full path step) lies within 2 SEM of the
+    exact drift velocity of the sublattice walk with the field-shifted
+    rates (oracle.diffusion.exact_drift_velocity)."""
+    from oracle.diffusion import exact_drift_velocity
+    w = workloads.corundum(4)
+    table = w.build_table()
+    run = w.make_run(table)
+    fam = B.Family(run, table)
+    kT = temp * constants.K2AUTEMP
+    d = np.asarray(direction, dtype=float) @ run.material.lattice_matrix
+    d /= np.linalg.norm(d)
+    field = 1e7 * constants.VPM2AUFIELD * d
+    rate = B.expected_rate_per_carrier(fam, kT, field)
+    dt = HOPS / rate / ROWS
+    n_traj = 256
+    occs, words = rng.seeded_starts(
+        seed, n_traj, [fam.hop_sites] * n_traj, [1] * n_traj)
+    trajs = [dict(family=0, n_carriers=1, carrier_q=-1.0, field=field,
+                  kT=kT, time_interval=dt, n_path=2 * ROWS + 1,
+                  max_hops=HOPS) for _ in range(n_traj)]
+    bt = B.KmcBatch([fam], "f64", "f32")
+    bt.setup(trajs, occs, words, waves_per_cu=4)
+    bt.run()
+    res = bt.results()
+    assert np.all(res["hops"] == HOPS)
+    v = []
+    for k in range(n_traj):
+        rows = bt.rows(k)
+        last = int(res["start"][k]) - 1
+        v.append((rows[last] - rows[0]) @ d / (last * dt))
+    bt.close()
+    v = np.array(v)
+    mean, sem = v.mean(), v.std() / np.sqrt(n_traj)
+    exact = exact_drift_velocity(run.nbr_site, run.nbr_vec, run.nbr_lambda,
+                                 run.nbr_vab, run.nbr_count,
+                                 np.arange(w.n_sites) % 12, run.material.vn,
+                                 kT, field) @ d
+    print(f"T={temp} K along {direction}: v = {mean:.5e} +- {sem:.2e} "
+          f"bohr/au; exact {exact:.5e}")
+    assert abs(mean - exact) <= 2 * sem
+    assert abs(exact) > 5 * sem   # the check resolves the drift
