@@ -217,6 +217,22 @@ int pycd_kmc_advance(pycd_kmc* s, const double* uniforms, int64_t n_pairs,
                      int64_t max_hops, double* hop_time, int32_t* hop_proc,
                      int64_t* hops_out);
 
+/* pycd_kmc_advance without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No reference counterpart.
+ * The same hop loop and results as pycd_kmc_advance, but every buffer must
+ * be device memory, and the log(u2) staging and the status words live in a
+ * fixed device array of the library (one per device, 8 MiB, kept until the
+ * process ends) instead of session buffers grown on demand: at most 2^20
+ * uniform pairs per call (PYCD_ERR_ARG beyond).  Calls on one device share
+ * that array: calls that overlap on a device race.  The session itself
+ * (pycd_kmc_create, pycd_kmc_begin) is set up beforehand as usual; this
+ * call does not update pycd_kmc_last_ms. */
+int pycd_kmc_advance_static(pycd_kmc* s, const double* uniforms,
+                            int64_t n_pairs, int64_t max_hops,
+                            double* hop_time, int32_t* hop_proc,
+                            int64_t* hops_out);
+
 /* Device time (ms, HIP events on the context's stream) of the kernel of
  * the last pycd_kmc_advance call. */
 double pycd_kmc_last_ms(const pycd_kmc* s);

@@ -96,6 +96,8 @@ _SIGNATURES = {
     "pycd_kmc_variant": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "pycd_kmc_advance": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P,
                                    C.POINTER(C.c_int64)]),
+    "pycd_kmc_advance_static": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P,
+                                          _P, C.POINTER(C.c_int64)]),
     "pycd_kmc_read": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "pycd_msd": (C.c_int, [_P, _P, C.c_int64, C.c_int64, C.c_int64,
                            C.c_int64, _P]),
@@ -443,6 +445,20 @@ class KmcSession:
         _check(self.ctx, rc, "pycd_kmc_advance")
         self.last_made = made.value
         return lib().pycd_kmc_last_ms(self.h)
+
+    def advance_static(self, dev_uniforms, n_pairs, max_hops=None,
+                       dev_times=None, dev_procs=None, offset_pairs=0):
+        """pycd_kmc_advance_static: device buffers only, nothing allocated;
+        uniforms from pair `offset_pairs` of `dev_uniforms`.  Returns
+        (rc, hops made)."""
+        made = C.c_int64(0)
+        u = C.c_void_p(dev_uniforms.ptr.value + 16 * int(offset_pairs))
+        rc = lib().pycd_kmc_advance_static(
+            self.h, u, int(n_pairs), int(max_hops or 0),
+            dev_times.ptr if dev_times is not None else None,
+            dev_procs.ptr if dev_procs is not None else None, C.byref(made))
+        _check(self.ctx, rc, "pycd_kmc_advance_static")
+        return rc, made.value
 
     def read(self):
         pb = self.pb

@@ -19,9 +19,15 @@ struct pycd_ctx {
   int cu_count = 0;
   double wall_clock_hz = 1e8;  // rate of the kernels' wall_clock64()
   std::vector<uint32_t> simd_keys;  // hardware keys of the SIMDs (probed once)
+  // device address of the static scratch of pycd_kmc_advance_static
+  double* static_scratch = nullptr;
 };
 
 namespace pycd {
+
+// Device address of the plain hop loop's static scratch on `device` (the
+// current device), resolved once per device (kmc_plain.hip).
+int kmc_static_scratch(int device, double** out);
 
 inline int fail(pycd_ctx* ctx, int code, const char* fmt, ...) {
   char buf[512];

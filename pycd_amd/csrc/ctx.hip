@@ -21,6 +21,11 @@ extern "C" int pycd_ctx_create(int device, pycd_ctx** out) {
     return PYCD_ERR_HIP;
   }
   ctx->cu_count = prop.multiProcessorCount;
+  if (pycd::kmc_static_scratch(device, &ctx->static_scratch) != PYCD_OK) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PYCD_ERR_HIP;
+  }
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) ==
           hipSuccess &&

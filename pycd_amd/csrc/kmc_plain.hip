@@ -25,6 +25,7 @@
 #include "wave.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 
 namespace pycd {
@@ -780,6 +781,20 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
   }
 }
 
+// Scratch of pycd_kmc_advance_static: log(u2) of up to kStaticPairs uniform
+// pairs, then the hop loop's two status words.  One copy per device (every
+// device the library runs on holds it until the process ends); host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// device (pycd_ctx_create resolves it once per device).  Calls that overlap
+// on one device share it and race.
+constexpr int64_t kStaticPairs = int64_t(1) << 20;
+__device__ __align__(256) double scratch[kStaticPairs + 32];
+
+__global__ void reset_status(int32_t* __restrict__ status) {
+  status[0] = -1;
+  status[1] = 0;
+}
+
 // phi_L[i] = sum_j A[i, j] q_L[j]: one wavefront per row.
 __global__ void __launch_bounds__(256)
 row_dot_kernel(const double* __restrict__ A, const double* __restrict__ q,
@@ -1304,6 +1319,32 @@ hipError_t launch_wave(bool compact, int npl, size_t smem, hipStream_t st,
 
 }  // namespace
 
+namespace pycd {
+
+int hop_loop(pycd_kmc* s, const double* dU, double* LU, int32_t* status_dev,
+             int64_t n_pairs, int64_t max_hops, double* dT, int32_t* dP,
+             bool timed, int64_t* made);
+
+// Device address of `scratch` on `device` (made current by the caller),
+// resolved once per device and kept.
+int kmc_static_scratch(int device, double** out) {
+  static std::mutex mu;
+  static std::vector<double*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (device < 0) return PYCD_ERR_ARG;
+  if (addr.size() <= size_t(device)) addr.resize(size_t(device) + 1, nullptr);
+  if (!addr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return PYCD_ERR_HIP;
+    addr[device] = static_cast<double*>(p);
+  }
+  *out = addr[device];
+  return PYCD_OK;
+}
+
+}  // namespace pycd
+
 extern "C" int pycd_kmc_advance(pycd_kmc* s, const double* uniforms,
                                 int64_t n_pairs, int64_t max_hops,
                                 double* hop_time, int32_t* hop_proc,
@@ -1339,6 +1380,35 @@ extern "C" int pycd_kmc_advance(pycd_kmc* s, const double* uniforms,
       PYCD_HIP(ctx, s->pbuf.alloc(size_t(std::max<int64_t>(n_pairs, 1))));
     dP = s->pbuf.ptr;
   }
+  if (s->lubuf.count < size_t(std::max<int64_t>(n_pairs, 1)))
+    PYCD_HIP(ctx, s->lubuf.alloc(size_t(std::max<int64_t>(n_pairs, 1))));
+  if (!s->ev0) {
+    PYCD_HIP(ctx, hipEventCreate(&s->ev0));
+    PYCD_HIP(ctx, hipEventCreate(&s->ev1));
+  }
+  int64_t made = 0;
+  const int rc = hop_loop(s, dU, s->lubuf.ptr, s->status.ptr, n_pairs,
+                          max_hops, dT, dP, true, &made);
+  if (t_host && made > 0)
+    PYCD_HIP(ctx, hipMemcpy(hop_time, dT, 8 * size_t(made),
+                            hipMemcpyDeviceToHost));
+  if (p_host && made > 0)
+    PYCD_HIP(ctx, hipMemcpy(hop_proc, dP, 4 * size_t(made),
+                            hipMemcpyDeviceToHost));
+  if (hops_out) *hops_out = made;
+  return rc;
+}
+
+namespace pycd {
+
+// The hop loop of pycd_kmc_advance and pycd_kmc_advance_static: log(u2) of
+// the chunk into LU, the kernel, its status into `status` (device) and back.
+// Allocates nothing.  `timed`: HIP events around it (created beforehand).
+int hop_loop(pycd_kmc* s, const double* dU, double* LU, int32_t* status_dev,
+             int64_t n_pairs, int64_t max_hops, double* dT, int32_t* dP,
+             bool timed, int64_t* made) {
+  pycd_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
   const pycd_kmc_problem& pb = s->pb;
   KmcDev d;
   d.A = s->A.ptr;
@@ -1372,50 +1442,71 @@ extern "C" int pycd_kmc_advance(pycd_kmc* s, const double* uniforms,
   d.epath = s->epath.ptr;
   d.dpath = s->dpath.ptr;
   d.sc = s->sc.ptr;
-  if (!s->ev0) {
-    PYCD_HIP(ctx, hipEventCreate(&s->ev0));
-    PYCD_HIP(ctx, hipEventCreate(&s->ev1));
-  }
-  if (s->lubuf.count < size_t(std::max<int64_t>(n_pairs, 1)))
-    PYCD_HIP(ctx, s->lubuf.alloc(size_t(std::max<int64_t>(n_pairs, 1))));
-  PYCD_HIP(ctx, hipEventRecord(s->ev0, st));
+  if (timed) PYCD_HIP(ctx, hipEventRecord(s->ev0, st));
+  reset_status<<<1, 1, 0, st>>>(status_dev);
   if (n_pairs > 0)
     log_u2_kernel<<<int(std::min<int64_t>((n_pairs + 255) / 256, 4096)), 256,
-                    0, st>>>(dU, n_pairs, s->lubuf.ptr);
+                    0, st>>>(dU, n_pairs, LU);
   const bool wave = s->variant == PYCD_KMC_WAVE ||
                     (s->variant == PYCD_KMC_AUTO && s->wave_ok);
   if (wave) {
     PYCD_HIP(ctx, launch_wave(s->compact, s->wave_npl, s->wave_smem, st, d,
-                              dU, s->lubuf.ptr, n_pairs, max_hops, dT, dP,
-                              s->status.ptr));
+                              dU, LU, n_pairs, max_hops, dT, dP, status_dev));
   } else if (s->compact) {
     kmc_plain_kernel<true><<<1, kThreads, s->smem, st>>>(
-        d, dU, s->lubuf.ptr, n_pairs, max_hops, dT, dP, s->status.ptr);
+        d, dU, LU, n_pairs, max_hops, dT, dP, status_dev);
   } else {
     kmc_plain_kernel<false><<<1, kThreads, s->smem, st>>>(
-        d, dU, s->lubuf.ptr, n_pairs, max_hops, dT, dP, s->status.ptr);
+        d, dU, LU, n_pairs, max_hops, dT, dP, status_dev);
   }
   PYCD_HIP(ctx, hipGetLastError());
-  PYCD_HIP(ctx, hipEventRecord(s->ev1, st));
+  if (timed) PYCD_HIP(ctx, hipEventRecord(s->ev1, st));
   int32_t status[2] = {-1, 0};
-  PYCD_HIP(ctx, hipMemcpyAsync(status, s->status.ptr, sizeof(status),
+  PYCD_HIP(ctx, hipMemcpyAsync(status, status_dev, sizeof(status),
                                hipMemcpyDeviceToHost, st));
   PYCD_HIP(ctx, hipStreamSynchronize(st));
-  {
+  if (timed) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
     s->last_ms = ms;
   }
-  const int64_t made = status[1];
-  if (t_host && made > 0)
-    PYCD_HIP(ctx, hipMemcpy(hop_time, dT, 8 * size_t(made),
-                            hipMemcpyDeviceToHost));
-  if (p_host && made > 0)
-    PYCD_HIP(ctx, hipMemcpy(hop_proc, dP, 4 * size_t(made),
-                            hipMemcpyDeviceToHost));
-  if (hops_out) *hops_out = made;
+  *made = status[1];
   if (status[0] < 0) return fail(ctx, PYCD_ERR_STATE, "kernel did not report");
   return status[0] == 1 ? PYCD_NEED_UNIFORMS : PYCD_OK;
+}
+
+}  // namespace pycd
+
+extern "C" int pycd_kmc_advance_static(pycd_kmc* s, const double* uniforms,
+                                       int64_t n_pairs, int64_t max_hops,
+                                       double* hop_time, int32_t* hop_proc,
+                                       int64_t* hops_out) {
+  if (!s) return PYCD_ERR_ARG;
+  pycd_ctx* ctx = s->ctx;
+  if (!s->begun) return fail(ctx, PYCD_ERR_STATE, "pycd_kmc_begin not called");
+  if (n_pairs < 0 || (n_pairs > 0 && !uniforms))
+    return fail(ctx, PYCD_ERR_ARG, "bad uniform buffer");
+  if (n_pairs > kStaticPairs)
+    return fail(ctx, PYCD_ERR_ARG,
+                "pycd_kmc_advance_static takes at most %lld uniform pairs "
+                "per call (its log(u2) scratch is a fixed device array); got "
+                "%lld",
+                (long long)kStaticPairs, (long long)n_pairs);
+  if (!ctx->static_scratch)
+    return fail(ctx, PYCD_ERR_STATE, "static scratch of the device unresolved");
+  DeviceGuard guard(ctx->device);
+  if ((n_pairs > 0 && !is_device_ptr(uniforms)) ||
+      (hop_time && !is_device_ptr(hop_time)) ||
+      (hop_proc && !is_device_ptr(hop_proc)))
+    return fail(ctx, PYCD_ERR_ARG,
+                "pycd_kmc_advance_static takes device buffers only");
+  double* LU = ctx->static_scratch;
+  int32_t* status = reinterpret_cast<int32_t*>(LU + kStaticPairs);
+  int64_t made = 0;
+  const int rc = hop_loop(s, uniforms, LU, status, n_pairs, max_hops,
+                          hop_time, hop_proc, false, &made);
+  if (hops_out) *hops_out = made;
+  return rc;
 }
 
 extern "C" int pycd_kmc_read(pycd_kmc* s, double* unwrapped, double* energy,
