@@ -183,15 +183,25 @@ __device__ inline void tile_coords_upper(int n, int tile, int& bi, int& bj) {
   bi = bj = 0;  // not reached: blockIdx.x < T
 }
 
-// Entry (i, j) of a tile and, off the diagonal block, its mirror (j, i):
-// the mirror's reciprocal sum is the same FP32/FP64 accumulation (the same
-// products in the same order), its real-space part is formed from (j, i).
+// Entry (i, j) of a tile and, off the diagonal block, its mirror (j, i),
+// which is the same number: the mirror's reciprocal sum is the same
+// accumulation (the same products in the same order), and its real-space
+// part comes from pos[i] - pos[j] = -(pos[j] - pos[i]) exactly, whose
+// fractional coordinates, round-half-even wrap and candidate images (the
+// kept offsets come in +/- pairs) are the negatives of those of (i, j), so
+// the squared minimum-image distance and everything after it are equal.
+// The FP64 epilogue is therefore formed once per pair.
 __device__ __forceinline__ void store_pair(double* __restrict__ A, int n, int i, int j,
                                   bool mirror, const double* __restrict__ pos,
                                   const MicGeometry& g, const EwaldScalars& es,
                                   double acc, unsigned& cnt) {
-  A[int64_t(i) * n + j] = assemble_entry(i, j, pos, g, es, acc, cnt);
-  if (mirror) A[int64_t(j) * n + i] = assemble_entry(j, i, pos, g, es, acc, cnt);
+  const unsigned before = cnt;
+  const double v = assemble_entry(i, j, pos, g, es, acc, cnt);
+  A[int64_t(i) * n + j] = v;
+  if (mirror) {
+    A[int64_t(j) * n + i] = v;
+    cnt += cnt - before;  // ordered pairs within the cutoff: (j, i) too
+  }
 }
 
 int launch_table(pycd_ctx* ctx, int precision, const double* pos, int n,

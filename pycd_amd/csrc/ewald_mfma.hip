@@ -143,9 +143,8 @@ table_mfma_f32(const float* __restrict__ Y, int n, int Kp,
     __syncthreads();
   }
   unsigned cnt = 0;
-  // entries (i, j), then the mirrored (j, i) of an off-diagonal tile, in two
-  // fully unrolled passes (one pass with both would not unroll, and the
-  // accumulators would leave the registers)
+  // entry (i, j) and, off the diagonal block, its mirror (j, i): the same
+  // number, formed once (store_pair)
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -155,23 +154,9 @@ table_mfma_f32(const float* __restrict__ Y, int n, int Kp,
         const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r;
         const int j = j0 + wc + 16 * y + (lane & 15);
         if (i < n && j < n)
-          A[int64_t(i) * n + j] =
-              assemble_entry(i, j, pos, g, es, double(acc[x][y][r]), cnt);
+          store_pair(A, n, i, j, mirror, pos, g, es, double(acc[x][y][r]),
+                     cnt);
       }
-  if (mirror) {
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-      for (int y = 0; y < 4; ++y)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r;
-          const int j = j0 + wc + 16 * y + (lane & 15);
-          if (i < n && j < n)
-            A[int64_t(j) * n + i] =
-                assemble_entry(j, i, pos, g, es, double(acc[x][y][r]), cnt);
-        }
-  }
   count_pairs(cnt, pairs);
 }
 
@@ -257,9 +242,8 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
     __syncthreads();
   }
   unsigned cnt = 0;
-  // entries (i, j), then the mirrored (j, i) of an off-diagonal tile, in two
-  // fully unrolled passes (one pass with both would not unroll, and the
-  // accumulators would leave the registers)
+  // entry (i, j) and, off the diagonal block, its mirror (j, i): the same
+  // number, formed once (store_pair)
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -269,23 +253,9 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
         const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r;
         const int j = j0 + wc + 16 * y + (lane & 15);
         if (i < n && j < n)
-          A[int64_t(i) * n + j] =
-              assemble_entry(i, j, pos, g, es, double(acc[x][y][r]), cnt);
+          store_pair(A, n, i, j, mirror, pos, g, es, double(acc[x][y][r]),
+                     cnt);
       }
-  if (mirror) {
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-      for (int y = 0; y < 4; ++y)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r;
-          const int j = j0 + wc + 16 * y + (lane & 15);
-          if (i < n && j < n)
-            A[int64_t(j) * n + i] =
-                assemble_entry(j, i, pos, g, es, double(acc[x][y][r]), cnt);
-        }
-  }
   count_pairs(cnt, pairs);
 }
 
