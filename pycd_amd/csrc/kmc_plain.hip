@@ -372,8 +372,75 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
 }
 
 // ---------------------------------------------------------------------------
+// Inputs of the one-wavefront kernel.  Per (site, slot) a 64-byte record:
+// the self term A[s, s] - A[s, n], lambda, V_ab, the field shift
+// 0.5 F.d and the hop vector d; per site (phi_L, E); per (site, slot) the
+// target and its packed code (0 without cell-0 rows).
+struct alignas(16) SlotRec {
+  double tself, lam, vab, shift;
+  double v[3];
+  double pad;
+};
+
+// What the one-wavefront kernel's hop loop reads, by value; state read only
+// before and after the loop comes from a KmcDev in device memory.
+struct WaveArgs {
+  const char* tab;       // the table (full, or the rows of cell 0)
+  uint32_t row_bytes;    // N sizeof(double)
+  int32_t uq;            // every carrier has charge q0
+  GeoPk gp;
+  const SlotRec* rec;
+  const double2* pe;
+  const int2* nbr2;
+  double* rows;
+  double* epath;
+  double* dpath;
+  const double* U;
+  const double* LU;
+  double* hop_time;
+  int32_t* hop_proc;
+  int32_t nc, z, mz, field_on;
+  double q0, nu, kT, dt;
+  int64_t P, n_pairs, max_hops;
+};
+
+// Records of every (site, slot) from the slot tables (the shift with the
+// workgroup kernel's arithmetic: products and sums rounded one by one).
+__global__ void pack_slot_records(int64_t NM, const double* __restrict__ tself,
+                                  const double* __restrict__ lam,
+                                  const double* __restrict__ vab,
+                                  const double* __restrict__ vec,
+                                  const int32_t* __restrict__ nbr,
+                                  const uint32_t* __restrict__ nbr_pk,
+                                  double f0, double f1, double f2,
+                                  int field_on, SlotRec* __restrict__ rec,
+                                  int2* __restrict__ nbr2) {
+#pragma clang fp contract(off)
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= NM) return;
+  SlotRec r;
+  r.tself = tself[k];
+  r.lam = lam[k];
+  r.vab = vab[k];
+  r.v[0] = vec[3 * k];
+  r.v[1] = vec[3 * k + 1];
+  r.v[2] = vec[3 * k + 2];
+  r.shift = field_on ? 0.5 * (f0 * r.v[0] + f1 * r.v[1] + f2 * r.v[2]) : 0.0;
+  r.pad = 0.0;
+  rec[k] = r;
+  nbr2[k] = make_int2(nbr[k], nbr_pk ? int(nbr_pk[k]) : 0);
+}
+
+__global__ void pack_site_pe(int64_t N, const double* __restrict__ phiL,
+                             const double* __restrict__ E,
+                             double2* __restrict__ pe) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < N) pe[k] = make_double2(phiL[k], E[k]);
+}
+
 // One-wavefront instance with a row cache (used when it fits: n_c <= 64,
-// n_c z <= 256, z <= 8 and the cache within the LDS of a workgroup).
+// n_c z <= 256, z <= 8, the cache within the LDS of a workgroup and the
+// table below 4 GiB).
 //
 // Rows: process row p (p < n_c z) is the target site of process p, site row
 // c is carrier c's site.  The cache holds G[c][row] = A[x_row, occ_c] for
@@ -389,43 +456,67 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
 //
 // Lane l owns processes l kNpl .. l kNpl + kNpl - 1 (the workgroup kernel's
 // split of the selection, so k_tot and the prefix are summed in the same
-// order) with their rate inputs in registers, and site row l (l < n_c).
-// One wavefront: no workgroup barriers; __syncthreads only waits for the
-// wavefront's own LDS operations.
+// order) with their rate inputs in registers, site row l and carrier l's
+// displacement, path base and drift (l < n_c).  One wavefront: no workgroup
+// barriers; __syncthreads only waits for the wavefront's own LDS
+// operations.  Cheap per-lane work is done by every lane and kept by a
+// select rather than branched around; gathers of lanes that have no entry
+// to update read a valid address and are dropped.
+template <typename T>
+using gmem = const __attribute__((address_space(1))) T*;
+
 template <bool kCompact, int kNpl>
 __global__ void __launch_bounds__(64)
-kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
-                const double* __restrict__ LU, int64_t n_pairs,
-                int64_t max_hops, double* __restrict__ hop_time,
-                int32_t* __restrict__ hop_proc, int32_t* __restrict__ status) {
-  constexpr int kB = 5;  // (1 + z) n_c <= 320 mover-row gathers per lane
+kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
+                int32_t* __restrict__ status) {
+  constexpr int kB = 5;  // (1 + z) n_c <= 320 mover-row gathers: 5 per lane
   extern __shared__ double smem[];
-  const int nc = d.nc, z = d.z, mz = d.max_z, nproc = nc * z;
+  const int nc = a.nc, z = a.z, nproc = nc * z;
   const int R = nproc + nc;
+  const int ncp = (nc + 1) >> 1;  // carrier pairs of the cache
   const int lane = threadIdx.x;
-  double* G = smem;                                   // nc x R, carrier-major
-  double* psis = G + size_t(nc) * R;                  // psi of the site rows
-  double* cqs = psis + nc;                            // carrier charges
-  double* disp = cqs + nc;                            // 3 nc
-  double* base = disp + 3 * nc;                       // 3 nc
-  double* drift = base + 3 * nc;                      // 3 nc
-  int32_t* occ = reinterpret_cast<int32_t*>(drift + 3 * nc);      // nc
-  uint32_t* opk = reinterpret_cast<uint32_t*>(occ + nc);          // nc
-  int32_t* tt = reinterpret_cast<int32_t*>(opk + nc);             // nproc z
-  uint32_t* ttpk = reinterpret_cast<uint32_t*>(tt + nproc * z);   // nproc z
-
-  // A[x, y] of the full table, or (compact) of the cell-0 row of x's
-  // sublattice at y moved by -cell(x)
-  auto entry = [&](int x, uint32_t xk, int y, uint32_t yk) -> double {
-    if constexpr (kCompact) {
-      return d.Ac[int64_t(xk >> 24) * d.N + rel_col(xk, yk, d.gp)];
-    } else {
-      return d.A[int64_t(x) * d.N + y];
-    }
+  // cache: carriers 2p and 2p + 1 of a row side by side, so that one
+  // 16-byte read serves two terms of a row's sum
+  double2* G = reinterpret_cast<double2*>(smem);          // ncp x R
+  double* psis = reinterpret_cast<double*>(G + size_t(ncp) * R);  // nc
+  double* cqs = psis + nc;                                // nc
+  int32_t* occ = reinterpret_cast<int32_t*>(cqs + nc);    // nc
+  uint32_t* opk = reinterpret_cast<uint32_t*>(occ + nc);  // nc
+  int32_t* tt = reinterpret_cast<int32_t*>(opk + nc);     // nproc z
+  uint32_t* ttpk = reinterpret_cast<uint32_t*>(tt + nproc * z);
+  auto gset = [&](int c, int row, double v) {
+    double* e = reinterpret_cast<double*>(G + size_t(c >> 1) * R + row);
+    e[c & 1] = v;
   };
+
+  // A[x, y]: the full table's row x, or (compact) the cell-0 row of x's
+  // sublattice at y moved by -cell(x); 32-bit byte offsets from the table's
+  // base (the host keeps the table below 4 GiB for this kernel)
+  auto entry = [&](int x, uint32_t xk, int y, uint32_t yk) -> double {
+    uint32_t off;
+    if constexpr (kCompact)
+      off = (xk >> 24) * a.row_bytes + rel_col(xk, yk, a.gp) * 8u;
+    else
+      off = uint32_t(x) * a.row_bytes + uint32_t(y) * 8u;
+    return *(gmem<double>)(a.tab + off);
+  };
+  // psi of a row: the FMA chain over carriers 0 .. n_c - 1
   auto row_sum = [&](int row) -> double {
     double acc = 0.0;
-    for (int c = 0; c < nc; ++c) acc = fma(cqs[c], G[size_t(c) * R + row], acc);
+    if (a.uq) {
+      const double q = a.q0;
+      for (int p = 0; p < ncp; ++p) {
+        const double2 v = G[size_t(p) * R + row];
+        acc = fma(q, v.x, acc);
+        if (2 * p + 1 < nc) acc = fma(q, v.y, acc);
+      }
+    } else {
+      for (int p = 0; p < ncp; ++p) {
+        const double2 v = G[size_t(p) * R + row];
+        acc = fma(cqs[2 * p], v.x, acc);
+        if (2 * p + 1 < nc) acc = fma(cqs[2 * p + 1], v.y, acc);
+      }
+    }
     return acc;
   };
 
@@ -435,37 +526,39 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
   bool pv[kNpl];
   double pqs[kNpl], pdphi[kNpl], ptq[kNpl], pdE[kNpl], plam[kNpl], pvab[kNpl],
       psh[kNpl], pvx[kNpl], pvy[kNpl], pvz[kNpl], pst[kNpl];
-  // the site-dependent part of process i's rate: slot j of site s, target n
+  // the site-dependent part of process i's rate: slot j of site s, whose
+  // target is n
   auto load_inputs = [&](int i, int s, int j, int n) {
-    const int64_t sj = int64_t(s) * mz + j;
-    pdphi[i] = d.phiL[n] - d.phiL[s];
-    ptq[i] = pqs[i] * d.tself[sj];
-    pdE[i] = d.E[n] - d.E[s];
-    plam[i] = d.lam[sj];
-    pvab[i] = d.vab[sj];
-    const double v0 = d.vec[3 * sj], v1 = d.vec[3 * sj + 1],
-                 v2 = d.vec[3 * sj + 2];
-    pvx[i] = v0;
-    pvy[i] = v1;
-    pvz[i] = v2;
-    double shift = 0.0;
-    if (d.field_on) {
-#pragma clang fp contract(off)
-      shift = 0.5 * (d.field[0] * v0 + d.field[1] * v1 + d.field[2] * v2);
-    }
-    psh[i] = shift;
+    const SlotRec& r = a.rec[int64_t(s) * a.mz + j];
+    const double2 ps = a.pe[s], pn = a.pe[n];
+    pdphi[i] = pn.x - ps.x;
+    ptq[i] = pqs[i] * r.tself;
+    pdE[i] = pn.y - ps.y;
+    plam[i] = r.lam;
+    pvab[i] = r.vab;
+    psh[i] = r.shift;
+    pvx[i] = r.v[0];
+    pvy[i] = r.v[1];
+    pvz[i] = r.v[2];
   };
 
   for (int c = lane; c < nc; c += 64) {
-    const int s = d.occ[c];
+    const int s = dv->occ[c];
     occ[c] = s;
-    opk[c] = kCompact ? d.pk[s] : 0u;
-    cqs[c] = d.cq[c];
+    opk[c] = kCompact ? dv->pk[s] : 0u;
+    cqs[c] = dv->cq[c];
   }
-  for (int k = lane; k < 3 * nc; k += 64) {
-    disp[k] = d.disp[k];
-    base[k] = d.base[k];
-    drift[k] = d.drift[k];
+  // carrier `lane`'s displacement since the last path row, path base and
+  // drift (n_c <= 64)
+  double dsp[3] = {0.0, 0.0, 0.0}, bse[3] = {0.0, 0.0, 0.0},
+         drf[3] = {0.0, 0.0, 0.0};
+  if (lane < nc) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      dsp[k] = dv->disp[3 * lane + k];
+      bse[k] = dv->base[3 * lane + k];
+      drf[k] = dv->drift[3 * lane + k];
+    }
   }
   __syncthreads();
   int last_i = -1;
@@ -482,10 +575,10 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
     if (pv[i]) {
       last_i = i;
       const int s = occ[pc[i]];
-      const int n = d.nbr[int64_t(s) * mz + pj[i]];
-      px[i] = n;
-      pxk[i] = kCompact ? d.pk[n] : 0u;
-      load_inputs(i, s, pj[i], n);
+      const int2 t = a.nbr2[int64_t(s) * a.mz + pj[i]];
+      px[i] = t.x;
+      pxk[i] = uint32_t(t.y);
+      load_inputs(i, s, pj[i], t.x);
     } else {
       pdphi[i] = ptq[i] = pdE[i] = pvab[i] = psh[i] = 0.0;
       plam[i] = 1.0;
@@ -496,35 +589,41 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
   for (int u = lane; u < nproc * z; u += 64) {
     const int p = u / z, k = u - p * z;
     const int c = p / z, j = p - c * z;
-    const int n = d.nbr[int64_t(occ[c]) * mz + j];
-    const int t = d.nbr[int64_t(n) * mz + k];
-    tt[u] = t;
-    ttpk[u] = kCompact ? d.pk[t] : 0u;
+    const int n = a.nbr2[int64_t(occ[c]) * a.mz + j].x;
+    const int2 t = a.nbr2[int64_t(n) * a.mz + k];
+    tt[u] = t.x;
+    ttpk[u] = uint32_t(t.y);
   }
-  // the cache, every (carrier, row)
+  // the cache, every (carrier, row); odd n_c leaves the last pair's second
+  // half zero
+  for (int u = lane; u < ncp * R; u += 64)
+    G[u] = make_double2(0.0, 0.0);
+  __syncthreads();
   for (int u = lane; u < nc * R; u += 64) {
     const int row = u / nc, c = u - row * nc;
     int x;
     uint32_t xk;
     if (row < nproc) {
       const int c2 = row / z, j2 = row - c2 * z;
-      x = d.nbr[int64_t(occ[c2]) * mz + j2];
-      xk = kCompact ? d.pk[x] : 0u;
+      const int2 t = a.nbr2[int64_t(occ[c2]) * a.mz + j2];
+      x = t.x;
+      xk = uint32_t(t.y);
     } else {
       x = occ[row - nproc];
       xk = opk[row - nproc];
     }
-    G[size_t(c) * R + row] = entry(x, xk, occ[c], opk[c]);
+    gset(c, row, entry(x, xk, occ[c], opk[c]));
   }
-  int sx = lane < nc ? occ[lane] : 0;
-  uint32_t sxk = lane < nc ? opk[lane] : 0u;
+  // site row of this lane (lanes past n_c keep carrier 0's, never stored)
+  int sx = occ[lane < nc ? lane : 0];
+  uint32_t sxk = opk[lane < nc ? lane : 0];
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kNpl; ++i)
     if (pv[i]) pst[i] = row_sum(lane * kNpl + i);
   if (lane < nc) psis[lane] = row_sum(nproc + lane);
   // the mover-row gathers this lane takes: row brl (0: the mover's site,
-  // 1 + j: its target j) at carrier bc
+  // 1 + j: its target j) at carrier bc; nb of the kB rounds hold any
   int brl[kB], bc[kB];
 #pragma unroll
   for (int k = 0; k < kB; ++k) {
@@ -532,24 +631,36 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
     brl[k] = u < (1 + z) * nc ? u / nc : -1;
     bc[k] = brl[k] >= 0 ? u - brl[k] * nc : 0;
   }
+  const int nb = ((1 + z) * nc + 63) / 64;
   // look-ahead load of this lane: target lk of the mover's new target lj
-  const int lj = lane < z * z ? lane / z : 0;
-  const int lk = lane < z * z ? lane - lj * z : 0;
+  const bool la_on = lane < z * z;
+  const int lj = la_on ? lane / z : 0;
+  const int lk = la_on ? lane - lj * z : 0;
   __syncthreads();
 
-  double t = d.sc->t, energy = d.sc->energy;
-  int64_t start = d.sc->start, hops = d.sc->hops;
-  int noselect = d.sc->no_select;
-  int stop = d.sc->finished ? 1 : 0;
+  const KmcScalars sc0 = *dv->sc;
+  double t = sc0.t, energy = sc0.energy;
+  int64_t start = sc0.start, hops = sc0.hops;
+  int noselect = sc0.no_select;
+  int stop = sc0.finished ? 1 : 0;
   int code = stop ? 0 : -1;
   int64_t h = 0;
   double u1c = 0.0, u2c = 0.0, luc = 0.0;
+  // hop log of the current block of 64 hops: lane l holds hop (h & ~63) + l
+  double tlog = 0.0;
+  int32_t plog = 0;
+  auto flush_log = [&](int64_t block, int count) {
+    if (lane < count) {
+      if (a.hop_time) a.hop_time[block + lane] = tlog;
+      if (a.hop_proc) a.hop_proc[block + lane] = plog;
+    }
+  };
   while (!stop) {
-    if (max_hops > 0 && hops >= max_hops) {
+    if (a.max_hops > 0 && hops >= a.max_hops) {
       code = 0;
       break;
     }
-    if (h >= n_pairs) {
+    if (h >= a.n_pairs) {
       code = 1;  // need uniforms
       break;
     }
@@ -557,10 +668,10 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
     if (hl == 0) {
       // lane l keeps the uniforms of hop h + l
       const int64_t q = h + lane;
-      if (q < n_pairs) {
-        u1c = U[2 * q];
-        u2c = U[2 * q + 1];
-        luc = LU[q];
+      if (q < a.n_pairs) {
+        u1c = a.U[2 * q];
+        u2c = a.U[2 * q + 1];
+        luc = a.LU[q];
       }
     }
     // ---- rates of this lane's processes
@@ -579,7 +690,7 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
         const double lam = plam[i];
         const double gs =
             (((lam + g0) * (lam + g0) / (4 * lam)) - pvab[i]) - psh[i];
-        kk[i] = d.nu * numpy_e_pow(-gs / d.kT);
+        kk[i] = a.nu * numpy_e_pow(-gs / a.kT);
         gg[i] = g0;
         ksum += kk[i];
       }
@@ -636,50 +747,25 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
     const double g_sel = read_lane(g_my, src);
     const double v_sel[3] = {read_lane(vx_my, src), read_lane(vy_my, src),
                              read_lane(vz_my, src)};
-    // ---- clock, energy and path bookkeeping (uniform)
-    int64_t end;
-    if (u2 > 0.0) {
-      t -= lu / k_tot;
-      const double q = t / d.dt;
-      end = q < 9.0e18 ? int64_t(q) : d.P;
-    } else {
-      t = INFINITY;
-      end = d.P;
-    }
-    energy += g_sel;
-    if (lane == 0) {
-      if (hop_time) hop_time[h] = t;
-      if (hop_proc) hop_proc[h] = sel;
-    }
-    const int64_t row_lo = start;
-    int64_t row_hi = start;
-    if (end >= start + 1) row_hi = end >= d.P ? d.P : end;
-    const int64_t dp_hi = min(end, d.P);
-    ++hops;
-    // loop condition of the reference: continue while end < P
-    stop = (row_hi > row_lo && row_hi >= d.P) ? 1 : 0;
 
     // ---- one round of loads: column m of the other rows, the mover's new
     // rows, the rate inputs of its new processes and their look-ahead
     const int tsel = sel * z;  // the mover's new targets: tt[tsel + j]
     double ga[kNpl];
 #pragma unroll
-    for (int i = 0; i < kNpl; ++i) {
-      ga[i] = 0.0;
-      if (pv[i] && pc[i] != m) ga[i] = entry(px[i], pxk[i], n, npk);
-    }
-    double gs_site = 0.0;
-    if (lane < nc && lane != m) gs_site = entry(sx, sxk, n, npk);
+    for (int i = 0; i < kNpl; ++i) ga[i] = entry(px[i], pxk[i], n, npk);
+    const double gs_site = entry(sx, sxk, n, npk);
     double gb[kB];
 #pragma unroll
     for (int k = 0; k < kB; ++k) {
       gb[k] = 0.0;
-      if (brl[k] >= 0) {
+      if (k < nb) {
+        const int rl = brl[k] > 0 ? brl[k] : 0;
         int x = n;
         uint32_t xk = npk;
-        if (brl[k] > 0) {
-          x = tt[tsel + brl[k] - 1];
-          xk = ttpk[tsel + brl[k] - 1];
+        if (rl > 0) {
+          x = tt[tsel + rl - 1];
+          xk = ttpk[tsel + rl - 1];
         }
         const int cc = bc[k];
         const int y = cc == m ? n : occ[cc];
@@ -696,19 +782,38 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
         load_inputs(i, n, pj[i], nn);
       }
     }
-    int la = 0;
-    uint32_t lak = 0u;
-    if (lane < z * z) {
-      const int64_t at = int64_t(tt[tsel + lj]) * mz + lk;
-      la = d.nbr[at];
-      lak = kCompact ? d.nbr_pk[at] : 0u;
+    const int2 la = a.nbr2[int64_t(tt[tsel + lj]) * a.mz + lk];
+
+    // ---- clock, energy and path bookkeeping (uniform), while the loads
+    // are in flight
+    int64_t end;
+    if (u2 > 0.0) {
+      t -= lu / k_tot;
+      const double qd = t / a.dt;
+      end = qd < 9.0e18 ? int64_t(qd) : a.P;
+    } else {
+      t = INFINITY;
+      end = a.P;
     }
-    // while the loads are in flight: the mover's displacement and drift
-    if (lane < 3) {
+    energy += g_sel;
+    tlog = lane == hl ? t : tlog;
+    plog = lane == hl ? sel : plog;
+    const int64_t row_lo = start;
+    int64_t row_hi = start;
+    if (end >= start + 1) row_hi = end >= a.P ? a.P : end;
+    const int64_t dp_hi = min(end, a.P);
+    ++hops;
+    // loop condition of the reference: continue while end < P
+    stop = (row_hi > row_lo && row_hi >= a.P) ? 1 : 0;
+    {
+      // the mover's displacement and drift, in lane m
 #pragma clang fp contract(off)
-      const double v = lane == 0 ? v_sel[0] : (lane == 1 ? v_sel[1] : v_sel[2]);
-      disp[3 * m + lane] += v;
-      if (d.field_on) drift[3 * m + lane] += v * k_sel;
+      const bool me = lane == m;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        dsp[k] = me ? dsp[k] + v_sel[k] : dsp[k];
+        if (a.field_on) drf[k] = me ? drf[k] + v_sel[k] * k_sel : drf[k];
+      }
     }
     if (lane == 0) {
       occ[m] = n;
@@ -718,21 +823,22 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
       sx = n;
       sxk = npk;
     }
-    // ---- the cache
+    // ---- the cache: column m of the other rows first, then the mover's
+    // rows (which overwrite what the first writes put in them)
 #pragma unroll
     for (int i = 0; i < kNpl; ++i)
-      if (pv[i] && pc[i] != m) G[size_t(m) * R + lane * kNpl + i] = ga[i];
-    if (lane < nc && lane != m) G[size_t(m) * R + nproc + lane] = gs_site;
+      if (pv[i]) gset(m, lane * kNpl + i, ga[i]);
+    if (lane < nc) gset(m, nproc + lane, gs_site);
 #pragma unroll
     for (int k = 0; k < kB; ++k) {
-      if (brl[k] >= 0) {
+      if (k < nb && brl[k] >= 0) {
         const int row = brl[k] == 0 ? nproc + m : m * z + brl[k] - 1;
-        G[size_t(bc[k]) * R + row] = gb[k];
+        gset(bc[k], row, gb[k]);
       }
     }
-    if (lane < z * z) {
-      tt[(m * z + lj) * z + lk] = la;
-      ttpk[(m * z + lj) * z + lk] = lak;
+    if (la_on) {
+      tt[(m * z + lj) * z + lk] = la.x;
+      ttpk[(m * z + lj) * z + lk] = uint32_t(la.y);
     }
     __syncthreads();
 #pragma unroll
@@ -742,40 +848,47 @@ kmc_wave_kernel(KmcDev d, const double* __restrict__ U,
     __syncthreads();
     // ---- path-step outputs, for a hop that ends one or more path steps
     if (row_hi > row_lo) {
-      for (int64_t r = row_lo + lane; r < dp_hi; r += 64) d.dpath[r] = g_sel;
-      const int w = 3 * nc;
-      for (int64_t idx = lane; idx < (row_hi - row_lo) * w; idx += 64) {
-        const int64_t r = row_lo + idx / w;
-        const int k = int(idx % w);
-        d.rows[r * w + k] = base[k] + disp[k];
-      }
-      for (int64_t r = row_lo + lane; r < row_hi; r += 64) d.epath[r] = energy;
-      __syncthreads();
-      for (int k = lane; k < 3 * nc; k += 64) {
-        base[k] += disp[k];
-        disp[k] = 0.0;
+      for (int64_t r = row_lo + lane; r < dp_hi; r += 64) a.dpath[r] = g_sel;
+      for (int64_t r = row_lo + lane; r < row_hi; r += 64) a.epath[r] = energy;
+      if (lane < nc) {
+        const int w = 3 * nc;
+        for (int64_t r = row_lo; r < row_hi; ++r)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) a.rows[r * w + 3 * lane + k] =
+              bse[k] + dsp[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          bse[k] += dsp[k];
+          dsp[k] = 0.0;
+        }
       }
       start = row_hi;
-      __syncthreads();
     }
+    if (hl == 63) flush_log(h - 63, 64);
     ++h;
     if (stop) code = 0;
   }
   // ---- write the state back
+  flush_log(h & ~int64_t(63), int(h & 63));
   __syncthreads();
-  for (int c = lane; c < nc; c += 64) d.occ[c] = occ[c];
-  for (int k = lane; k < 3 * nc; k += 64) {
-    d.disp[k] = disp[k];
-    d.base[k] = base[k];
-    d.drift[k] = drift[k];
+  for (int c = lane; c < nc; c += 64) dv->occ[c] = occ[c];
+  if (lane < nc) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      dv->disp[3 * lane + k] = dsp[k];
+      dv->base[3 * lane + k] = bse[k];
+      dv->drift[3 * lane + k] = drf[k];
+    }
   }
   if (lane == 0) {
-    d.sc->t = t;
-    d.sc->energy = energy;
-    d.sc->start = start;
-    d.sc->hops = hops;
-    d.sc->no_select = noselect;
-    if (stop) d.sc->finished = 1;
+    KmcScalars o = sc0;
+    o.t = t;
+    o.energy = energy;
+    o.start = start;
+    o.hops = hops;
+    o.no_select = noselect;
+    if (stop) o.finished = 1;
+    *dv->sc = o;
     status[0] = code;
     status[1] = int32_t(h);
   }
@@ -882,6 +995,12 @@ struct pycd_kmc {
   Geo geo{};
   DevBuf<double> Ac;
   DevBuf<uint32_t> pk, nbr_pk;
+  // one-wavefront kernel: slot records, (phi_L, E) per site, targets with
+  // their codes, the device copy of the kernel's state pointers
+  DevBuf<SlotRec> rec;
+  DevBuf<double2> pe;
+  DevBuf<int2> nbr2;
+  DevBuf<KmcDev> dev;
   double qLAqL = 0.0;
   size_t smem = 0;
   // one-wavefront row-cache kernel: whether it fits, its LDS, processes
@@ -890,6 +1009,8 @@ struct pycd_kmc {
   size_t wave_smem = 0;
   int wave_npl = 0;
   int32_t variant = PYCD_KMC_AUTO;
+  bool uniform_q = false;  // every carrier has charge q0
+  double q0 = 0.0;
   bool begun = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -1028,14 +1149,23 @@ hipError_t compact_setup(pycd_kmc* s, const std::vector<int32_t>& nb) {
   return hipSuccess;
 }
 
+// (phi_L, E) pairs of every site for the one-wavefront kernel.
+hipError_t pack_pe(pycd_kmc* s) {
+  const int64_t N = s->pb.n_sites;
+  pack_site_pe<<<int((N + 255) / 256), 256, 0, s->ctx->stream>>>(
+      N, s->phiL.ptr, s->E.ptr, s->pe.ptr);
+  return hipGetLastError();
+}
+
 // LDS of the one-wavefront kernel: the row cache (n_c x (n_c z + n_c)
-// doubles), site-row psi, charges, displacement/base/drift (11 n_c
-// doubles), carrier sites and their codes, and each process's look-ahead
-// (2 n_c z z words).  0 when the instance does not apply.
+// doubles, carriers in pairs), site-row psi and charges (2 n_c doubles),
+// carrier sites and their codes, and each process's look-ahead (2 n_c z z
+// words).  0 when the instance does not apply.
 size_t wave_lds_bytes(int nc, int z) {
   const int nproc = nc * z;
   if (nc > 64 || nproc > 256 || z > 8) return 0;
-  return sizeof(double) * (size_t(nc) * (nproc + nc) + 11 * size_t(nc)) +
+  const size_t ncp = (size_t(nc) + 1) / 2;
+  return 16 * ncp * (nproc + nc) + sizeof(double) * 2 * size_t(nc) +
          sizeof(int32_t) * (2 * size_t(nc) + 2 * size_t(nproc) * z);
 }
 
@@ -1134,11 +1264,38 @@ extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
   S_HIP(upload(s->vab.ptr, pb->nbr_vab, NM, st));
   S_HIP(upload(s->E.ptr, pb->site_energy, N, st));
   S_HIP(upload(s->cq.ptr, pb->carrier_q, nc, st));
+  {
+    std::vector<double> cq(nc);
+    if (is_device_ptr(pb->carrier_q)) {
+      S_HIP(hipMemcpy(cq.data(), pb->carrier_q, 8 * size_t(nc),
+                      hipMemcpyDeviceToHost));
+    } else {
+      std::memcpy(cq.data(), pb->carrier_q, 8 * size_t(nc));
+    }
+    s->q0 = cq[0];
+    s->uniform_q = true;
+    for (double v : cq) s->uniform_q = s->uniform_q && v == cq[0];
+  }
   self_term_kernel<<<int((NM + 255) / 256), 256, 0, st>>>(s->A.ptr, s->nbr.ptr,
                                                          N, mz, s->tself.ptr);
   S_HIP(hipGetLastError());
   S_HIP(lattice_terms(s));
   S_HIP(compact_setup(s, nb));
+  S_HIP(s->rec.alloc(NM));
+  S_HIP(s->nbr2.alloc(NM));
+  S_HIP(s->pe.alloc(size_t(N)));
+  S_HIP(s->dev.alloc(1));
+  {
+    const bool field_on =
+        pb->field[0] != 0.0 || pb->field[1] != 0.0 || pb->field[2] != 0.0;
+    pack_slot_records<<<int((NM + 255) / 256), 256, 0, st>>>(
+        int64_t(NM), s->tself.ptr, s->lam.ptr, s->vab.ptr, s->vec.ptr,
+        s->nbr.ptr, s->compact ? s->nbr_pk.ptr : nullptr, pb->field[0],
+        pb->field[1], pb->field[2], field_on ? 1 : 0, s->rec.ptr,
+        s->nbr2.ptr);
+    S_HIP(hipGetLastError());
+    S_HIP(pack_pe(s));
+  }
   {
     int lds_max = 0;
     S_HIP(hipDeviceGetAttribute(&lds_max,
@@ -1146,7 +1303,10 @@ extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
                                 ctx->device));
     s->wave_smem = wave_lds_bytes(nc, z);
     s->wave_npl = (nc * z + 63) / 64;
-    s->wave_ok = s->wave_smem > 0 && s->wave_smem <= size_t(lds_max);
+    // the kernel addresses the table with 32-bit byte offsets
+    const double tab_bytes = 8.0 * double(s->compact ? s->geo.nu : N) * N;
+    s->wave_ok = s->wave_smem > 0 && s->wave_smem <= size_t(lds_max) &&
+                 tab_bytes < 4294967296.0;
   }
 #undef S_HIP
   *out = s;
@@ -1195,6 +1355,7 @@ extern "C" int pycd_kmc_set_sites(pycd_kmc* s, const double* lattice_q,
   PYCD_HIP(ctx, upload(s->qL.ptr, lattice_q, N, st));
   PYCD_HIP(ctx, upload(s->E.ptr, site_energy, N, st));
   PYCD_HIP(ctx, lattice_terms(s));
+  PYCD_HIP(ctx, pack_pe(s));
   return PYCD_OK;
 }
 
@@ -1269,52 +1430,41 @@ extern "C" int pycd_kmc_begin(pycd_kmc* s, const int64_t* occupancy,
 namespace {
 
 template <bool kCompact, int kNpl>
-hipError_t launch_wave_t(size_t smem, hipStream_t st, const KmcDev& d,
-                         const double* U, const double* LU, int64_t n_pairs,
-                         int64_t max_hops, double* T, int32_t* P,
-                         int32_t* status) {
+hipError_t launch_wave_t(size_t smem, hipStream_t st, const WaveArgs& wa,
+                         const KmcDev* dv, int32_t* status) {
   if (smem > 65536) {
     hipError_t e = hipFuncSetAttribute(
         reinterpret_cast<const void*>(&kmc_wave_kernel<kCompact, kNpl>),
         hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
     if (e != hipSuccess) return e;
   }
-  kmc_wave_kernel<kCompact, kNpl>
-      <<<1, 64, smem, st>>>(d, U, LU, n_pairs, max_hops, T, P, status);
+  kmc_wave_kernel<kCompact, kNpl><<<1, 64, smem, st>>>(wa, dv, status);
   return hipGetLastError();
 }
 
 template <bool kCompact>
 hipError_t launch_wave_c(int npl, size_t smem, hipStream_t st,
-                         const KmcDev& d, const double* U, const double* LU,
-                         int64_t n_pairs, int64_t max_hops, double* T,
-                         int32_t* P, int32_t* status) {
+                         const WaveArgs& wa, const KmcDev* dv,
+                         int32_t* status) {
   switch (npl) {
     case 1:
-      return launch_wave_t<kCompact, 1>(smem, st, d, U, LU, n_pairs, max_hops,
-                                        T, P, status);
+      return launch_wave_t<kCompact, 1>(smem, st, wa, dv, status);
     case 2:
-      return launch_wave_t<kCompact, 2>(smem, st, d, U, LU, n_pairs, max_hops,
-                                        T, P, status);
+      return launch_wave_t<kCompact, 2>(smem, st, wa, dv, status);
     case 3:
-      return launch_wave_t<kCompact, 3>(smem, st, d, U, LU, n_pairs, max_hops,
-                                        T, P, status);
+      return launch_wave_t<kCompact, 3>(smem, st, wa, dv, status);
     case 4:
-      return launch_wave_t<kCompact, 4>(smem, st, d, U, LU, n_pairs, max_hops,
-                                        T, P, status);
+      return launch_wave_t<kCompact, 4>(smem, st, wa, dv, status);
     default:
       return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_wave(bool compact, int npl, size_t smem, hipStream_t st,
-                       const KmcDev& d, const double* U, const double* LU,
-                       int64_t n_pairs, int64_t max_hops, double* T,
-                       int32_t* P, int32_t* status) {
-  return compact ? launch_wave_c<true>(npl, smem, st, d, U, LU, n_pairs,
-                                       max_hops, T, P, status)
-                 : launch_wave_c<false>(npl, smem, st, d, U, LU, n_pairs,
-                                        max_hops, T, P, status);
+                       const WaveArgs& wa, const KmcDev* dv,
+                       int32_t* status) {
+  return compact ? launch_wave_c<true>(npl, smem, st, wa, dv, status)
+                 : launch_wave_c<false>(npl, smem, st, wa, dv, status);
 }
 
 }  // namespace
@@ -1450,8 +1600,36 @@ int hop_loop(pycd_kmc* s, const double* dU, double* LU, int32_t* status_dev,
   const bool wave = s->variant == PYCD_KMC_WAVE ||
                     (s->variant == PYCD_KMC_AUTO && s->wave_ok);
   if (wave) {
-    PYCD_HIP(ctx, launch_wave(s->compact, s->wave_npl, s->wave_smem, st, d,
-                              dU, LU, n_pairs, max_hops, dT, dP, status_dev));
+    WaveArgs wa;
+    wa.tab = reinterpret_cast<const char*>(s->compact ? s->Ac.ptr : s->A.ptr);
+    wa.row_bytes = uint32_t(pb.n_sites) * uint32_t(sizeof(double));
+    wa.gp = d.gp;
+    wa.rec = s->rec.ptr;
+    wa.pe = s->pe.ptr;
+    wa.nbr2 = s->nbr2.ptr;
+    wa.rows = d.rows;
+    wa.epath = d.epath;
+    wa.dpath = d.dpath;
+    wa.U = dU;
+    wa.LU = LU;
+    wa.hop_time = dT;
+    wa.hop_proc = dP;
+    wa.nc = pb.n_carriers;
+    wa.z = pb.z;
+    wa.mz = pb.max_z;
+    wa.field_on = d.field_on;
+    wa.uq = s->uniform_q ? 1 : 0;
+    wa.q0 = s->q0;
+    wa.nu = pb.nu;
+    wa.kT = pb.kT;
+    wa.dt = pb.time_interval;
+    wa.P = pb.n_path;
+    wa.n_pairs = n_pairs;
+    wa.max_hops = max_hops;
+    PYCD_HIP(ctx, hipMemcpyAsync(s->dev.ptr, &d, sizeof(KmcDev),
+                                 hipMemcpyHostToDevice, st));
+    PYCD_HIP(ctx, launch_wave(s->compact, s->wave_npl, s->wave_smem, st, wa,
+                              s->dev.ptr, status_dev));
   } else if (s->compact) {
     kmc_plain_kernel<true><<<1, kThreads, s->smem, st>>>(
         d, dU, LU, n_pairs, max_hops, dT, dP, status_dev);
