@@ -30,6 +30,15 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# workloads built by the GPU legs that the CPU legs time the oracle on
+_KEEP = {}
+
+
+def _json_default(o):
+    """numpy scalars and arrays in the JSON line"""
+    if hasattr(o, "tolist"):
+        return o.tolist()
+    raise TypeError(f"not JSON serialisable: {type(o).__name__}")
 
 
 def parse():
@@ -37,13 +46,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["batched", "precision"],
+    ap.add_argument("--mode", choices=["batched", "precision", "plain"],
                     default="batched")
     ap.add_argument("--precision-edge", type=int, default=10)
     ap.add_argument("--precision-hops", type=int, default=1_000_000)
     ap.add_argument("--n-traj", type=int, default=4096)
     ap.add_argument("--hops", type=int, default=100_000)
     ap.add_argument("--plain-hops", type=int, default=1_000_000)
+    ap.add_argument("--plain-variant", choices=["auto", "wave", "workgroup"],
+                    default="auto")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="default run: leave out the precision form")
     ap.add_argument("--waves-per-cu", type=int, default=0,
                     help="0: autotune")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -185,8 +198,11 @@ def run_plain(args, log):
         n_path=int(run.t_final / run.time_interval) + 1,
         cells=run.system_size)
     sess = _native.KmcSession(pb)
-    log(f"plain: hop loop reads {sess.table_rows()} table rows of "
-        f"{w.n_sites}")
+    if args.plain_variant != "auto":
+        sess.set_variant(args.plain_variant)
+    variant = sess.variant()
+    log(f"plain: {variant} kernel; hop loop reads {sess.table_rows()} "
+        f"table rows of {w.n_sites}")
     occs, words = seeded_starts(2, 1, [run.hop_site_lists["electron"]], [16])
     state = (3, tuple(int(v) for v in words[0]), None)
     n = args.plain_hops
@@ -197,20 +213,35 @@ def run_plain(args, log):
     hops = sess.read()["hops"]
     sess.close()
     dev.free()
-    # algorithmic bytes per hop of the plain kernel: the n_c (1 + z) x n_c
-    # table gathers of psi, 11 doubles per process (slot tables, phi_L, E,
-    # hop vector) and the two uniforms
     nc = 16
     z = int(np.max(run.nbr_count))
-    per_hop = 8 * nc * (1 + z) * nc + 8 * 11 * nc * z + 16
+    per_hop = plain_bytes_per_hop(variant, nc, z)
     gbs = per_hop * hops / (ms / 1e3) / 1e9
+    _KEEP["plain"] = (run, table)
     return {"hops": hops, "ms": ms, "hops_per_s": hops / (ms / 1e3),
             "n_sites": w.n_sites, "n_carriers": nc, "z": z,
+            "variant": variant,
             "roofline": {"bound": "hbm", "achieved": gbs,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS,
-                         "kernel": "kmc_plain_kernel",
+                         "kernel": ("kmc_wave_kernel" if variant == "wave"
+                                    else "kmc_plain_kernel"),
                          "algorithmic_bytes_per_hop": per_hop}}
+
+
+def plain_bytes_per_hop(variant, nc, z):
+    """Bytes the plain kernel's algorithm reads per hop.
+
+    wave (row cache in LDS): (1 + z)(2 n_c - 1) table entries (column m of
+    the other rows, the mover's 1 + z new rows), 10 doubles of rate inputs
+    for each of the mover's z new processes (phi_L and E at both ends, the
+    self term, lambda, V_ab, hop vector), 2 z^2 words of look-ahead slot
+    tables and 24 B of uniforms (u1, u2, log u2).
+    workgroup: n_c (1 + z) x n_c table entries (every row gathered again),
+    11 doubles per process and 16 B of uniforms."""
+    if variant == "wave":
+        return 8 * (1 + z) * (2 * nc - 1) + 80 * z + 8 * z * z + 24
+    return 8 * nc * (1 + z) * nc + 8 * 11 * nc * z + 16
 
 
 def run_serve(args, log):
@@ -259,6 +290,7 @@ def run_precision(args, log):
     from pycd_amd.rng import seeded_starts
     n = args.precision_edge
     w = workloads.corundum(n, m_kcut=n, n_carriers=32, t_final=1e-8)
+    _KEEP["precision"] = w
     pos = w.neighbors.bulk_sites.cell_coordinates
     out = {"n_sites": int(w.n_sites)}
     tables = {}
@@ -390,12 +422,47 @@ def cpu_worker(spec, budget):
     print(json.dumps({"hops": int(hops), "seconds": el}), flush=True)
 
 
+def _oracle_spec(path, table_path, run, lattice_q, nc, field, nu, kT, dt,
+                 n_path, seed, hop_sites):
+    np.savez(path, table=table_path, lattice_q=lattice_q,
+             count=run.nbr_count, nbr=run.nbr_site, vec=run.nbr_vec,
+             lam=run.nbr_lambda, vab=run.nbr_vab,
+             energy=run.system_relative_energies,
+             carrier_q=np.full(nc, -1.0), field=np.asarray(field), nu=nu,
+             kT=kT, dt=dt, n_path=n_path, seed=int(seed),
+             hop_sites=np.asarray(hop_sites), nc=nc)
+
+
+def _run_workers(args_list, entry, budget, log, what):
+    """One worker process (one thread) per spec, all at once; each prints
+    {"hops"/"rows": n, "seconds": s} as JSON.  Returns the parsed lines."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1")
+    procs = [subprocess.Popen(
+        [sys.executable, "-c", f"import bench; bench.{entry}({a})"],
+        cwd=str(Path(__file__).resolve().parent), env=env,
+        stdout=subprocess.PIPE, text=True) for a in args_list]
+    log(f"cpu {what}: {len(procs)} workers started")
+    out = []
+    for i, p in enumerate(procs):
+        try:
+            so, _ = p.communicate(timeout=4 * budget + 60)
+        except subprocess.TimeoutExpired:
+            p.kill()
+            p.communicate()
+            log(f"cpu {what} worker {i} timed out; left out")
+            continue
+        if p.returncode != 0:
+            raise RuntimeError(f"cpu worker exited with {p.returncode}")
+        out.append(json.loads(so.strip().splitlines()[-1]))
+    return out
+
+
 def cpu_all_cores(args, trajs, fams, cores, log):
     """The oracle on `cores` sampled trajectories at once, one worker
     process (one thread) per core.  Every trajectory of the workload makes
     the same number of hops, so cores working through it run at
     cores / mean(time per hop) over the sample."""
-    import subprocess
     import tempfile
     rng_ = np.random.default_rng(98)
     sample = rng_.choice(len(trajs), size=min(cores, len(trajs)),
@@ -404,7 +471,7 @@ def cpu_all_cores(args, trajs, fams, cores, log):
     # the host has it
     shm = "/dev/shm" if os.path.isdir("/dev/shm") else None
     with tempfile.TemporaryDirectory(dir=shm) as tmp:
-        tables, procs = {}, []
+        tables, specs = {}, []
         for k in sample:
             t = trajs[k]
             f = t["family"]
@@ -412,49 +479,142 @@ def cpu_all_cores(args, trajs, fams, cores, log):
             if f not in tables:
                 tables[f] = os.path.join(tmp, f"table{f}.npy")
                 np.save(tables[f], np.ascontiguousarray(fam.table))
-            run = fam.run
             spec = os.path.join(tmp, f"traj{k}.npz")
-            np.savez(spec, table=tables[f], lattice_q=fam.lattice_q,
-                     count=run.nbr_count, nbr=run.nbr_site, vec=run.nbr_vec,
-                     lam=run.nbr_lambda, vab=run.nbr_vab,
-                     energy=run.system_relative_energies,
-                     carrier_q=np.full(t["n_carriers"], -1.0),
-                     field=np.asarray(t["field"]), nu=fam.nu, kT=t["kT"],
-                     dt=t["time_interval"], n_path=t["n_path"], seed=int(k),
-                     hop_sites=np.asarray(fam.hop_sites),
-                     nc=t["n_carriers"])
-            env = dict(os.environ, OMP_NUM_THREADS="1",
-                       OPENBLAS_NUM_THREADS="1")
-            procs.append(subprocess.Popen(
-                [sys.executable, "-c",
-                 f"import bench; bench.cpu_worker({spec!r}, "
-                 f"{args.cpu_seconds!r})"],
-                cwd=str(Path(__file__).resolve().parent), env=env,
-                stdout=subprocess.PIPE, text=True))
-        log(f"cpu all cores: {len(procs)} workers started")
-        per_hop, hops = [], 0
-        for i, p in enumerate(procs):
-            try:
-                out, _ = p.communicate(timeout=4 * args.cpu_seconds + 60)
-            except subprocess.TimeoutExpired:
-                p.kill()
-                p.communicate()
-                log(f"cpu worker {i} timed out; left out")
-                continue
-            if p.returncode != 0:
-                raise RuntimeError(f"cpu worker exited with {p.returncode}")
-            r = json.loads(out.strip().splitlines()[-1])
-            per_hop.append(r["seconds"] / max(r["hops"], 1))
-            hops += r["hops"]
-            log(f"cpu worker {i}: {r['hops']} hops in {r['seconds']:.1f} s")
+            _oracle_spec(spec, tables[f], fam.run, fam.lattice_q,
+                         t["n_carriers"], t["field"], fam.nu, t["kT"],
+                         t["time_interval"], t["n_path"], k, fam.hop_sites)
+            specs.append(f"{spec!r}, {args.cpu_seconds!r}")
+        res = _run_workers(specs, "cpu_worker", args.cpu_seconds, log,
+                           "batched")
+    per_hop = [r["seconds"] / max(r["hops"], 1) for r in res]
+    hops = sum(r["hops"] for r in res)
     total = len(per_hop) / float(np.mean(per_hop))
     log(f"cpu all cores: {len(per_hop)} workers, {total:.1f} hops/s")
     return total, hops, len(per_hop)
 
 
+def cpu_plain(args, cores, log):
+    """The oracle on the plain configuration (8x8x8, 6144 sites, 16
+    electrons, 1e6 V/m): one trajectory on one core, then `cores`
+    trajectories (seeds 2, 3, ...) at once, one per core."""
+    import random
+    import tempfile
+    from threadpoolctl import threadpool_limits
+    from oracle import kmc, seeding
+    run, table = _KEEP["plain"]
+    budget = args.cpu_seconds / 2
+    pb = kmc.Problem(table, run.base_charge_config()[:, 0], run.nbr_count,
+                     run.nbr_site, run.nbr_vec, run.nbr_lambda, run.nbr_vab,
+                     run.system_relative_energies, [-1.0] * 16,
+                     run.electric_field, run.material.vn, run.temp,
+                     run.time_interval,
+                     int(run.t_final / run.time_interval) + 1)
+    gen = random.Random()
+    gen.setstate(seeding.trajectory_states(2, 1)[0])
+    occ = seeding.initial_occupancy(gen, run.hop_site_lists["electron"], 16)
+    with threadpool_limits(1):
+        hops, el = _oracle_rate(pb, occ, gen, budget)
+    out = {"one_core": hops / el, "one_core_sample": f"{hops} hops in "
+                                                     f"{el:.1f} s"}
+    log(f"cpu plain one core: {hops} hops in {el:.1f} s")
+    if cores > 1 and not args.cpu_single:
+        shm = "/dev/shm" if os.path.isdir("/dev/shm") else None
+        with tempfile.TemporaryDirectory(dir=shm) as tmp:
+            tpath = os.path.join(tmp, "table.npy")
+            np.save(tpath, np.ascontiguousarray(table))
+            specs = []
+            for c in range(cores):
+                spec = os.path.join(tmp, f"plain{c}.npz")
+                _oracle_spec(spec, tpath, run, run.base_charge_config()[:, 0],
+                             16, run.electric_field, run.material.vn,
+                             run.temp, run.time_interval,
+                             int(run.t_final / run.time_interval) + 1, 2 + c,
+                             run.hop_site_lists["electron"])
+                specs.append(f"{spec!r}, {budget!r}")
+            res = _run_workers(specs, "cpu_worker", budget, log, "plain")
+        per_hop = [r["seconds"] / max(r["hops"], 1) for r in res]
+        out["all_cores"] = len(per_hop) / float(np.mean(per_hop))
+        out["cores"] = len(per_hop)
+        log(f"cpu plain all cores: {out['all_cores']:.1f} hops/s")
+    return out
+
+
+def table_worker(spec, n_rows):
+    """Rows of the oracle's FP64 pair table at the precision configuration
+    from the .npz `spec`; prints rows and seconds as JSON."""
+    from oracle import ewald
+    z = np.load(spec, allow_pickle=False)
+    rows = [int(r) for r in z["rows"][:n_rows]]
+    t0 = time.perf_counter()
+    ewald.table_rows(z["pos"], rows, z["lattice"], z["size"], z["pbc"],
+                     float(z["alpha"]), float(z["r_cut"]), float(z["k_cut"]),
+                     float(z["eps"]))
+    print(json.dumps({"rows": len(rows),
+                      "seconds": time.perf_counter() - t0}), flush=True)
+
+
+def cpu_table(args, cores, log):
+    """Build time of the precision configuration's FP64 table (12000 sites,
+    K_half = 4918) by the oracle, from sampled rows: seconds per row on one
+    core, and rows per second with one worker per core, each scaled to the
+    full N rows."""
+    import tempfile
+    w = _KEEP["precision"]
+    s = w.system
+    n = int(w.n_sites)
+    shm = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    out = {"n_sites": n}
+    with tempfile.TemporaryDirectory(dir=shm) as tmp:
+        rng_ = np.random.default_rng(7)
+        specs = []
+        for c in range(max(cores, 1)):
+            spec = os.path.join(tmp, f"rows{c}.npz")
+            np.savez(spec, pos=w.neighbors.bulk_sites.cell_coordinates,
+                     rows=rng_.choice(n, size=64, replace=False),
+                     lattice=w.material.lattice_matrix, size=w.size,
+                     pbc=np.asarray(w.neighbors.pbc), alpha=s.alpha,
+                     r_cut=s.r_cut, k_cut=s.k_cut,
+                     eps=w.material.dielectric_constant)
+            specs.append(spec)
+        # one core: a few rows, timed in a worker of its own (one thread)
+        one = _run_workers([f"{specs[0]!r}, 6"], "table_worker", 30, log,
+                           "table one core")[0]
+        per_row = one["seconds"] / one["rows"]
+        out["one_core_s_per_row"] = per_row
+        out["one_core_full_table_s"] = per_row * n
+        log(f"cpu table one core: {per_row:.3f} s per row, "
+            f"{per_row * n:.0f} s for {n} rows")
+        if cores > 1 and not args.cpu_single:
+            k = max(2, int(min(64, args.cpu_seconds / 2 / per_row)))
+            res = _run_workers([f"{sp!r}, {k}" for sp in specs],
+                               "table_worker", k * per_row * 4 + 30, log,
+                               "table all cores")
+            rate = sum(r["rows"] / r["seconds"] for r in res)
+            out["all_cores_rows_per_s"] = rate
+            out["all_cores_full_table_s"] = n / rate
+            out["cores"] = len(res)
+            log(f"cpu table all cores: {n / rate:.0f} s for {n} rows")
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(args, trajs, fams, log):
     """NumPy oracle on a bounded sample of the batched workload: one core,
-    then one trajectory per host core at once."""
+    then one trajectory per host core at once; beside it the plain
+    configuration and the precision configuration's table build."""
+    from threadpoolctl import threadpool_limits
     rng_ = np.random.default_rng(99)
     sample = rng_.choice(len(trajs), size=min(8, len(trajs)), replace=False)
     budget = args.cpu_seconds / len(sample)
@@ -462,7 +622,8 @@ def cpu_baseline(args, trajs, fams, log):
     for k in sample:
         t = trajs[k]
         pb, occ, gen = _oracle_inputs(t, fams[t["family"]], k)
-        hops, el = _oracle_rate(pb, occ, gen, budget)
+        with threadpool_limits(1):
+            hops, el = _oracle_rate(pb, occ, gen, budget)
         spent += el
         done_hops += hops
         per_hop.append(el / max(hops, 1))
@@ -488,6 +649,12 @@ def cpu_baseline(args, trajs, fams, log):
             f"single_core = "
             f"{len(sample)} trajectories one after another on one core "
             f"({done_hops} hops in {spent:.1f} s)"))
+    out["cpu_model"] = cpu_model()
+    out["host_cores"] = len(os.sched_getaffinity(0))
+    if "plain" in _KEEP:
+        out["plain_cfg2"] = cpu_plain(args, cores, log)
+    if "precision" in _KEEP:
+        out["table_cfg3"] = cpu_table(args, cores, log)
     return out
 
 
@@ -501,10 +668,6 @@ def main():
         sys.exit(2)
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    if args.mode == "precision":
-        print(json.dumps({"mode": "precision",
-                          "result": run_precision(args, log)}))
-        return
     if args.copy_calibration > 0:
         from pycd_amd import _native
         n = args.copy_calibration // 8
@@ -515,6 +678,15 @@ def main():
         a.free()
         b.free()
         log(f"calibration: 3 device copies of {n * 8} bytes")
+    if args.mode == "precision":
+        print(json.dumps({"mode": "precision",
+                          "result": run_precision(args, log)},
+                         default=_json_default))
+        return
+    if args.mode == "plain":
+        print(json.dumps({"mode": "plain", "result": run_plain(args, log)},
+                         default=_json_default))
+        return
     info, trajs, fams = run_batched(args, log)
     log(f"batched: {info['hops_per_s']:.4e} hops/s, {info['ms_per_step']:.1f}"
         f" ms/step")
@@ -528,6 +700,9 @@ def main():
         log(f"serve: {serve['hops_per_s']:.4e} hops/s, p50 "
             f"{serve['p50_latency_ms']:.0f} ms, p99 "
             f"{serve['p99_latency_ms']:.0f} ms")
+    precision = None
+    if not args.no_precision:
+        precision = run_precision(args, log)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, trajs, fams, log)
@@ -565,9 +740,10 @@ def main():
         "longest_trajectory_ms": info["longest_trajectory_ms"],
         "resident_waves_per_cu": info["resident_waves_per_cu"],
         "roofline": info["roofline"],
+        "precision": precision,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
+    print(json.dumps(line, default=_json_default))
 
 
 if __name__ == "__main__":

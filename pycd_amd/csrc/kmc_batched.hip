@@ -55,7 +55,7 @@ struct SubRec {       // hop slot j of sublattice s
   double lam;         // lambda (Hartree)
   double vab;         // V_ab (Hartree)
   double tself;       // A[s, s] - A[s, n]
-  double dphi;        // phi_L[n] - phi_L[s]
+  double dphi;        // q_L . (A[n, :] - A[s, :])
   double dE;          // E[n] - E[s]
   double vec[3];      // hop vector (bohr)
 };
@@ -155,11 +155,6 @@ __device__ inline double uni(double v) {
   return __longlong_as_double((int64_t(hi) << 32) | lo);
 }
 
-__device__ inline double swap_f64(double* p, double v) {
-  return __longlong_as_double(static_cast<long long>(
-      atomicExch(reinterpret_cast<unsigned long long*>(p),
-                 static_cast<unsigned long long>(__double_as_longlong(v)))));
-}
 
 
 // The family's arrays are reached through pointers read from memory, which
@@ -171,6 +166,12 @@ using gptr = const __attribute__((address_space(1))) T*;
 template <typename T>
 __device__ inline gptr<T> as_global(const T* p) {
   return (gptr<T>)p;
+}
+template <typename T>
+using gmut = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ inline gmut<T> as_global_mut(T* p) {
+  return (gmut<T>)p;
 }
 
 __device__ inline SubRec load_rec(const FamDev& f, uint32_t k) {
@@ -585,15 +586,17 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         td.field[0] != 0.0 || td.field[1] != 0.0 || td.field[2] != 0.0;
 
     // resumable per-trajectory state in global memory.  Displacement and
-    // drift are accumulated there by atomic adds that nothing waits for
-    // (lanes 0-2, one hop after another) and read back by atomic exchange
-    // or add, which see every earlier add of this wavefront at L2.
+    // drift live there while the trajectory runs: only this wavefront
+    // reads and writes them, lanes 0-2 read the mover's in the hop's round
+    // of loads and store the sums (plain stores; an atomic add per hop
+    // would be carried to memory each time).
     uint32_t* mts = a.mts + int64_t(tr) * 625;
     uint32_t* socc = a.st_occ + int64_t(tr) * kMaxCarriers;
     double* svec = a.st_vec + int64_t(tr) * kVecStride;
-    double* sdisp = svec;
+    const gmut<double> svec_g = as_global_mut(svec);
+    const gmut<double> sdisp = svec_g;
     double* sbase = svec + 3 * kMaxCarriers;
-    double* sdrift = svec + 6 * kMaxCarriers;
+    const gmut<double> sdrift = svec_g + 6 * kMaxCarriers;
     double* spsi = a.st_psi + int64_t(tr) * kPsiStride;
 
     // ---- load or start the state
@@ -611,8 +614,8 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     }
     if (!resume)
       for (int k = lane; k < w; k += 64) {
-        swap_f64(sdisp + k, 0.0);
-        swap_f64(sdrift + k, 0.0);
+        sdisp[k] = 0.0;
+        sdrift[k] = 0.0;
         sbase[k] = 0.0;
         rows[k] = 0.0;  // path row 0: the initial positions
       }
@@ -690,8 +693,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     // boundary when it is handed over
     auto save = [&](bool done_now) {
       for (int k = lane; k < w; k += 64)
-        a.drift[int64_t(tr) * 3 * kMaxCarriers + k] =
-            atomicAdd(sdrift + k, 0.0);
+        a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = sdrift[k];
       for (int k = lane; k < 624; k += 64) mts[k] = mt[k];
       for (int c = lane; c < nc; c += 64) socc[c] = occ[c];
       for (int r = lane; r < R; r += 64) spsi[r] = psi[r];
@@ -937,15 +939,22 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       // both entries of every slot are loaded before any is used, so that
       // the slots' gathers are in flight together
       TT an[kRPer], ao[kRPer];
+      // the slot sites of this lane's rows, read together (one LDS wait,
+      // not one per row)
+      uint32_t xs[kRPer];
+#pragma unroll
+      for (int i = 0; i < kRPer; ++i) {
+        const int r = lane_h + 64 * i;
+        xs[i] = sites[r < R_h ? r : 0];
+      }
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
         const int r = lane_h + 64 * i;
         an[i] = TT(0);
         ao[i] = TT(0);
         if (r < R_h) {
-          const uint32_t x = sites[r];
-          an[i] = tab.at(rn, rel_col(key_new, x, gp));
-          ao[i] = tab.at(ro, rel_col(key_old, x, gp));
+          an[i] = tab.at(rn, rel_col(key_new, xs[i], gp));
+          ao[i] = tab.at(ro, rel_col(key_old, xs[i], gp));
         }
       }
       // fresh sums of the new slots (not the one that returns to s_old):
@@ -963,8 +972,16 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
             fent[k] = tab.at(tab.row(nsel[k] >> 24), rel_col(nsel[k], o, gp));
         }
       }
-      // hop vector of carrier m, one component per lane 0..2
-      const double vec_own = lane_h < 3 ? ptab[rec_old].vec[lane_h] : 0.0;
+      // hop vector of carrier m, and its displacement (lanes 0..2) and
+      // drift (lanes 3..5, with a field) so far, one component per lane
+      const int hc = lane_h < 3 ? lane_h : lane_h - 3;
+      const bool mv = lane_h < 3 || (lane_h < 6 && field_on);
+      const int64_t vo = (lane_h < 3 ? 0 : 6 * kMaxCarriers) + 3 * m + hc;
+      double vec_own = 0.0, v_acc = 0.0;
+      if (mv) {
+        vec_own = ptab[rec_old].vec[hc];
+        v_acc = svec_g[vo];
+      }
       // ---- apply (process m's old entries are dead once it is selected)
       if (lane_h < z_h) pinfo[m * z_h + lane_h] = info_new;
 #pragma unroll
@@ -972,9 +989,9 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         const int r = lane_h + 64 * i;
         if (r < R_h) psi[r] += q * (double(an[i]) - double(ao[i]));
       }
-      if (lane_h < 3) {
-        atomicAdd(sdisp + 3 * m + lane_h, vec_own);
-        if (field_on) atomicAdd(sdrift + 3 * m + lane_h, vec_own * ksel);
+      if (mv) {
+#pragma clang fp contract(off)
+        svec_g[vo] = v_acc + (lane_h < 3 ? vec_own : vec_own * ksel);
       }
       __syncthreads();
       const double v_new = uni(psi[m * zp + 1 + j]);
@@ -1006,7 +1023,8 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       if (end >= start + 1) {
         if (end >= P) end = P;
         for (int k = lane_h; k < w; k += 64) {
-          const double v = sbase[k] + swap_f64(sdisp + k, 0.0);
+          const double v = sbase[k] + sdisp[k];
+          sdisp[k] = 0.0;
           for (int32_t r = start; r < end; ++r) rows[int64_t(r) * w + k] = v;
           sbase[k] = v;
         }
@@ -1328,7 +1346,7 @@ struct pycd_batch {
   std::vector<BatchFamily*> fams;
   DevBuf<FamDev> d_fams;
   int table_prec = 0;  // 0 FP64, 1 FP32
-  int64_t refresh = 1024;
+  int64_t refresh = 1024;   // set from the rate precision at creation
   int rate_prec = 0;   // 0 FP64, 1 FP32
   double nu = 0.0;
   // per-run buffers

@@ -48,6 +48,7 @@ struct KmcDev {
   const uint32_t* nbr_pk;
   GeoPk gp;
   const double* phiL;
+  const double* dphi;  // q_L . (A[n, :] - A[s, :]) per (site, slot)
   const int32_t* nbr;
   const double* vec;
   const double* lam;
@@ -129,7 +130,7 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
     const int n = d.nbr[sj];
     pn[p] = n;
     if constexpr (kCompact) pnpk[p] = d.pk[n];
-    pdphi[p] = d.phiL[n] - d.phiL[s];
+    pdphi[p] = d.dphi[sj];
     ptq[p] = d.cq[c] * d.tself[sj];
     pdE[p] = d.E[n] - d.E[s];
     plam[p] = d.lam[sj];
@@ -374,12 +375,13 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
 // ---------------------------------------------------------------------------
 // Inputs of the one-wavefront kernel.  Per (site, slot) a 64-byte record:
 // the self term A[s, s] - A[s, n], lambda, V_ab, the field shift
-// 0.5 F.d and the hop vector d; per site (phi_L, E); per (site, slot) the
-// target and its packed code (0 without cell-0 rows).
+// 0.5 F.d, the hop vector d and the lattice part q_L . (A[n, :] - A[s, :])
+// of the hop's potential difference; per site (phi_L, E); per (site, slot)
+// the target and its packed code (0 without cell-0 rows).
 struct alignas(16) SlotRec {
   double tself, lam, vab, shift;
   double v[3];
-  double pad;
+  double dphi;
 };
 
 // What the one-wavefront kernel's hop loop reads, by value; state read only
@@ -407,6 +409,7 @@ struct WaveArgs {
 // Records of every (site, slot) from the slot tables (the shift with the
 // workgroup kernel's arithmetic: products and sums rounded one by one).
 __global__ void pack_slot_records(int64_t NM, const double* __restrict__ tself,
+                                  const double* __restrict__ dphi,
                                   const double* __restrict__ lam,
                                   const double* __restrict__ vab,
                                   const double* __restrict__ vec,
@@ -426,7 +429,7 @@ __global__ void pack_slot_records(int64_t NM, const double* __restrict__ tself,
   r.v[1] = vec[3 * k + 1];
   r.v[2] = vec[3 * k + 2];
   r.shift = field_on ? 0.5 * (f0 * r.v[0] + f1 * r.v[1] + f2 * r.v[2]) : 0.0;
-  r.pad = 0.0;
+  r.dphi = dphi[k];
   rec[k] = r;
   nbr2[k] = make_int2(nbr[k], nbr_pk ? int(nbr_pk[k]) : 0);
 }
@@ -531,7 +534,7 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
   auto load_inputs = [&](int i, int s, int j, int n) {
     const SlotRec& r = a.rec[int64_t(s) * a.mz + j];
     const double2 ps = a.pe[s], pn = a.pe[n];
-    pdphi[i] = pn.x - ps.x;
+    pdphi[i] = r.dphi;
     ptq[i] = pqs[i] * r.tself;
     pdE[i] = pn.y - ps.y;
     plam[i] = r.lam;
@@ -935,6 +938,55 @@ __global__ void self_term_kernel(const double* __restrict__ A,
   tself[idx] = n >= 0 ? A[s * N + s] - A[s * N + n] : 0.0;
 }
 
+// Lattice part of the potential difference of each hop, the reference's
+// q . (A[n, :] - A[s, :]) restricted to the lattice charges: every product
+// q_L[y] (A[n, y] - A[s, y]) rounded as NumPy forms it, their sum kept as
+// an unevaluated pair (hi, lo) (Knuth's TwoSum) and rounded once at the
+// end.  Forming it as phi_L[n] - phi_L[s] instead would carry about an ulp
+// of phi_L, a sum of order 10 Ha, into every rate.  One wavefront per
+// (site, slot); 0 for empty slots.
+__device__ inline void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+__global__ void __launch_bounds__(256)
+slot_dphi_kernel(const double* __restrict__ A, const double* __restrict__ q,
+                 const int32_t* __restrict__ nbr, int64_t N, int max_z,
+                 double* __restrict__ dphi) {
+#pragma clang fp contract(off)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t NM = N * max_z;
+  for (int64_t k = int64_t(blockIdx.x) * 4 + wave; k < NM;
+       k += int64_t(gridDim.x) * 4) {
+    const int n = nbr[k];
+    if (n < 0) {
+      if (lane == 0) dphi[k] = 0.0;
+      continue;
+    }
+    const double* rs = A + (k / max_z) * N;
+    const double* rn = A + int64_t(n) * N;
+    double hi = 0.0, lo = 0.0;
+    for (int64_t y = lane; y < N; y += 64) {
+      const double p = q[y] * (rn[y] - rs[y]);
+      double e;
+      two_sum(hi, p, hi, e);
+      lo += e;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double h2 = __shfl_xor(hi, off, 64);
+      const double l2 = __shfl_xor(lo, off, 64);
+      double e;
+      two_sum(hi, h2, hi, e);
+      lo = lo + l2 + e;
+    }
+    if (lane == 0) dphi[k] = hi + lo;
+  }
+}
+
 // Initial energy pieces: e[0] = q_L . phi_L is precomputed on the host side
 // of the session; this kernel adds the carrier terms
 // 2 sum_c q_c phi_L[occ_c] + sum_{c,c'} q_c q_c' A[occ_c, occ_c'].
@@ -984,7 +1036,7 @@ using namespace pycd;
 struct pycd_kmc {
   pycd_ctx* ctx = nullptr;
   pycd_kmc_problem pb{};
-  DevBuf<double> A, phiL, vec, lam, vab, tself, E, cq, qL;
+  DevBuf<double> A, phiL, dphi, vec, lam, vab, tself, E, cq, qL;
   DevBuf<int32_t> nbr, occ, status;
   DevBuf<double> disp, base, drift, rows, epath, dpath, tmp;
   DevBuf<KmcScalars> sc;
@@ -1036,14 +1088,21 @@ int check_problem(pycd_ctx* ctx, const pycd_kmc_problem* pb) {
   return PYCD_OK;
 }
 
-// phi_lattice = A q_lattice and q_lattice^T A q_lattice from the resident
-// table and lattice charges.
+// phi_lattice = A q_lattice, the lattice part of every hop's potential
+// difference and q_lattice^T A q_lattice from the resident table and
+// lattice charges.
 hipError_t lattice_terms(pycd_kmc* s) {
   hipStream_t st = s->ctx->stream;
   const int64_t N = s->pb.n_sites;
   const int blocks = int(std::min<int64_t>((N + 3) / 4, 8192));
   row_dot_kernel<<<blocks, 256, 0, st>>>(s->A.ptr, s->qL.ptr, N, s->phiL.ptr);
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t NM = N * s->pb.max_z;
+  slot_dphi_kernel<<<int(std::min<int64_t>((NM + 3) / 4, 16384)), 256, 0,
+                     st>>>(s->A.ptr, s->qL.ptr, s->nbr.ptr, N, s->pb.max_z,
+                           s->dphi.ptr);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   dot_kernel<<<1, 256, 0, st>>>(s->qL.ptr, s->phiL.ptr, N, s->tmp.ptr);
   e = hipGetLastError();
@@ -1149,11 +1208,22 @@ hipError_t compact_setup(pycd_kmc* s, const std::vector<int32_t>& nb) {
   return hipSuccess;
 }
 
-// (phi_L, E) pairs of every site for the one-wavefront kernel.
-hipError_t pack_pe(pycd_kmc* s) {
+// Slot records, targets and (phi_L, E) pairs of every site for the
+// one-wavefront kernel, from the slot tables and the lattice terms.
+hipError_t pack_wave_inputs(pycd_kmc* s) {
+  hipStream_t st = s->ctx->stream;
   const int64_t N = s->pb.n_sites;
-  pack_site_pe<<<int((N + 255) / 256), 256, 0, s->ctx->stream>>>(
-      N, s->phiL.ptr, s->E.ptr, s->pe.ptr);
+  const int64_t NM = N * s->pb.max_z;
+  const double* f = s->pb.field;
+  const bool field_on = f[0] != 0.0 || f[1] != 0.0 || f[2] != 0.0;
+  pack_slot_records<<<int((NM + 255) / 256), 256, 0, st>>>(
+      NM, s->tself.ptr, s->dphi.ptr, s->lam.ptr, s->vab.ptr, s->vec.ptr,
+      s->nbr.ptr, s->compact ? s->nbr_pk.ptr : nullptr, f[0], f[1], f[2],
+      field_on ? 1 : 0, s->rec.ptr, s->nbr2.ptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  pack_site_pe<<<int((N + 255) / 256), 256, 0, st>>>(N, s->phiL.ptr,
+                                                     s->E.ptr, s->pe.ptr);
   return hipGetLastError();
 }
 
@@ -1238,6 +1308,7 @@ extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
   const int64_t P = pb->n_path;
   S_HIP(s->A.alloc(size_t(N) * N));
   S_HIP(s->phiL.alloc(N));
+  S_HIP(s->dphi.alloc(NM));
   S_HIP(s->qL.alloc(N));
   S_HIP(s->nbr.alloc(NM));
   S_HIP(s->vec.alloc(NM * 3));
@@ -1285,17 +1356,7 @@ extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
   S_HIP(s->nbr2.alloc(NM));
   S_HIP(s->pe.alloc(size_t(N)));
   S_HIP(s->dev.alloc(1));
-  {
-    const bool field_on =
-        pb->field[0] != 0.0 || pb->field[1] != 0.0 || pb->field[2] != 0.0;
-    pack_slot_records<<<int((NM + 255) / 256), 256, 0, st>>>(
-        int64_t(NM), s->tself.ptr, s->lam.ptr, s->vab.ptr, s->vec.ptr,
-        s->nbr.ptr, s->compact ? s->nbr_pk.ptr : nullptr, pb->field[0],
-        pb->field[1], pb->field[2], field_on ? 1 : 0, s->rec.ptr,
-        s->nbr2.ptr);
-    S_HIP(hipGetLastError());
-    S_HIP(pack_pe(s));
-  }
+  S_HIP(pack_wave_inputs(s));
   {
     int lds_max = 0;
     S_HIP(hipDeviceGetAttribute(&lds_max,
@@ -1355,7 +1416,7 @@ extern "C" int pycd_kmc_set_sites(pycd_kmc* s, const double* lattice_q,
   PYCD_HIP(ctx, upload(s->qL.ptr, lattice_q, N, st));
   PYCD_HIP(ctx, upload(s->E.ptr, site_energy, N, st));
   PYCD_HIP(ctx, lattice_terms(s));
-  PYCD_HIP(ctx, pack_pe(s));
+  PYCD_HIP(ctx, pack_wave_inputs(s));
   return PYCD_OK;
 }
 
@@ -1567,6 +1628,7 @@ int hop_loop(pycd_kmc* s, const double* dU, double* LU, int32_t* status_dev,
   d.nbr_pk = s->nbr_pk.ptr;
   d.gp = geo_pk(s->compact ? s->geo : Geo{1, 1, 1, 1});
   d.phiL = s->phiL.ptr;
+  d.dphi = s->dphi.ptr;
   d.nbr = s->nbr.ptr;
   d.vec = s->vec.ptr;
   d.lam = s->lam.ptr;

@@ -51,6 +51,25 @@ def oracle_problem(run, table=None, field=None):
         int(run.t_final / run.time_interval) + 1)
 
 
+def cell0_table(table, cells, n_unit):
+    """The N x N table a cell-periodic family reads: entry (a, b) is the
+    row of a's sublattice in cell 0 at b moved by -cell(a), with sites
+    numbered ((cx ny + cy) nz + cz) n_unit + sub.  It equals `table` to
+    rounding (the family's set-up checks that); running the oracle on it
+    compares the batched kernel with the reference's loop on the very
+    entries the kernel reads."""
+    nx, ny, nz = (int(v) for v in cells)
+    idx = np.arange(nx * ny * nz * n_unit)
+    sub = idx % n_unit
+    c = idx // n_unit
+    cz, cy, cx = c % nz, (c // nz) % ny, c // (nz * ny)
+    dx = (cx[None, :] - cx[:, None]) % nx
+    dy = (cy[None, :] - cy[:, None]) % ny
+    dz = (cz[None, :] - cz[:, None]) % nz
+    col = ((dx * ny + dy) * nz + dz) * n_unit + sub[None, :]
+    return np.ascontiguousarray(table[sub[:, None], col])
+
+
 DOPING_SHIFTS = [0.25, -0.05, 0.01]  # eV at hop shells 0, 1, 2 of a dopant
 
 

@@ -240,12 +240,13 @@ def test_served_monoclinic_jobs_vs_oracle():
     """The serving form's own lattice (monoclinic, 4 sites per cell, hops to
     2.886 and 2.902 A) through the server in 257-hop slices with FP64
     rates: every job's hop count, clock, path rows and energy equal the
-    oracle's trajectory on the same seed and time grid
+    oracle's trajectory on the same seed and time grid, run on the table
+    entries the served family reads (its cell-0 rows)
     (PyCD/core.py:2647-2861)."""
     import random
     from oracle import kmc, seeding
     from pycd_amd import constants
-    from tests.systems import oracle_problem
+    from tests.systems import cell0_table, oracle_problem
     rng = np.random.default_rng(21)
     specs = [dict(edge=int(rng.integers(4, 7)),
                   n_carriers=int(rng.integers(1, 9)),
@@ -263,8 +264,14 @@ def test_served_monoclinic_jobs_vs_oracle():
     n_path = srv.n_path
     srv.close()
     assert len({s["edge"] for s in specs}) > 1
+    cell0 = {}
     for j, s in zip(jobs, specs):
         run, table = monoclinic_family(s["edge"])
+        if s["edge"] not in cell0:
+            cell0[s["edge"]] = cell0_table(
+                table, run.system_size,
+                table.shape[0] // int(np.prod(run.system_size)))
+        table = cell0[s["edge"]]
         nc = s["n_carriers"]
         field = (np.array([0.0, 0.0, 1.0]) * s["field_vpm"]
                  * constants.VPM2AUFIELD if s["field_vpm"] else np.zeros(3))
@@ -281,7 +288,7 @@ def test_served_monoclinic_jobs_vs_oracle():
         r = j.result
         assert r["hops"] == ora["hops"] > 0, j.id
         assert abs(r["t"] - ora["time_data"][-1]) <= \
-            1e-10 * ora["time_data"][-1], j.id
+            1e-12 * ora["time_data"][-1], j.id
         assert r["start"] == ora["start"], j.id
         assert np.max(np.abs(r["rows"] - ora["unwrapped"])) < 1e-8, j.id
         scale = max(1.0, abs(ora["energy_now"]))
