@@ -60,6 +60,7 @@ struct KmcDev {
   int32_t nc, z, max_z, field_on;
   double field[3];
   double nu, kT, dt;
+  double nkt;  // -1 / kT
   int64_t P;
   // state
   int32_t* occ;
@@ -102,13 +103,13 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
   double* drift = base + 3 * nc;      // 3 nc
   // per process p = c z + j (carrier c, slot j of its site): the values of
   // its rate that depend only on the site, refreshed when c hops
-  double* pdphi = drift + 3 * nc;     // phi_L[n] - phi_L[s]
+  double* pdphi = drift + 3 * nc;     // q_L . (A[n, :] - A[s, :])
   double* ptq = pdphi + nproc;        // q_c tself[s, j]
   double* pdE = ptq + nproc;          // E[n] - E[s]
   double* plam = pdE + nproc;         // lambda
-  double* pvab = plam + nproc;        // V_ab
-  double* pshift = pvab + nproc;      // field . hop vector / 2
-  double* pvec = pshift + nproc;      // hop vector (3 nproc)
+  double* pi4l = plam + nproc;        // 1 / (4 lambda)
+  double* pvs = pi4l + nproc;         // V_ab + field . hop vector / 2
+  double* pvec = pvs + nproc;         // hop vector (3 nproc)
   int32_t* pn = reinterpret_cast<int32_t*>(pvec + 3 * nproc);  // target n
   int32_t* occ = pn + nproc;          // nc
   // compact instance: packed codes of the carriers' sites and of each
@@ -116,6 +117,7 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
   uint32_t* opk = reinterpret_cast<uint32_t*>(occ + nc);  // nc
   uint32_t* pnpk = opk + nc;                              // nproc
   __shared__ double sh_t, sh_energy, sh_dg;
+  __shared__ double sh_exp[64];  // 2^(j/64)
   __shared__ int64_t sh_start, sh_end, sh_hops, sh_row_lo, sh_row_hi,
       sh_dp_hi;
   __shared__ int32_t sh_stop, sh_noselect, sh_mover;
@@ -133,15 +135,19 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
     pdphi[p] = d.dphi[sj];
     ptq[p] = d.cq[c] * d.tself[sj];
     pdE[p] = d.E[n] - d.E[s];
-    plam[p] = d.lam[sj];
-    pvab[p] = d.vab[sj];
-    double shift = 0.0;
-    if (d.field_on) {
+    {
+      // the same arithmetic as the one-wavefront kernel's slot records
 #pragma clang fp contract(off)
-      shift = 0.5 * (d.field[0] * d.vec[3 * sj] + d.field[1] * d.vec[3 * sj + 1] +
-                     d.field[2] * d.vec[3 * sj + 2]);
+      const double lam = d.lam[sj];
+      double shift = 0.0;
+      if (d.field_on)
+        shift = 0.5 * (d.field[0] * d.vec[3 * sj] +
+                       d.field[1] * d.vec[3 * sj + 1] +
+                       d.field[2] * d.vec[3 * sj + 2]);
+      plam[p] = lam;
+      pi4l[p] = 1.0 / (4 * lam);
+      pvs[p] = d.vab[sj] + shift;
     }
-    pshift[p] = shift;
     pvec[3 * p] = d.vec[3 * sj];
     pvec[3 * p + 1] = d.vec[3 * sj + 1];
     pvec[3 * p + 2] = d.vec[3 * sj + 2];
@@ -155,6 +161,7 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
     base[k] = d.base[k];
     drift[k] = d.drift[k];
   }
+  if (tid < 64) sh_exp[tid] = __builtin_bit_cast(double, kExp2J64[tid]);
   if (tid == 0) {
     sh_t = d.sc->t;
     sh_energy = d.sc->energy;
@@ -240,10 +247,7 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
           pdphi[p] + (psi[c * (1 + z) + 1 + j] - psi[c * (1 + z)]);
       const double term02 = ptq[p];
       const double g0 = 2 * qs * (term01 + term02) + pdE[p];
-      const double lam = plam[p];
-      const double gs = (((lam + g0) * (lam + g0) / (4 * lam)) - pvab[p]) -
-                        pshift[p];
-      kk[p] = d.nu * numpy_e_pow(-gs / d.kT);
+      kk[p] = marcus_rate(g0, plam[p], pi4l[p], pvs[p], d.nkt, d.nu, sh_exp);
       dg[p] = g0;
     }
     __syncthreads();
@@ -252,18 +256,21 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
       const int per = (nproc + 63) / 64;
       const int p0 = lane * per;
       const int p1 = min(p0 + per, nproc);
+      // the first process whose cumulative rate exceeds u1 k_tot (the
+      // reference compares the cumulative sum of k / k_tot with u1: the
+      // same choice but where u1 lies within an ulp of a boundary)
       double ksum = 0.0;
       for (int p = p0; p < p1; ++p) ksum += kk[p];
-      const double k_tot = read_lane(wave_incl_scan(ksum), 63);
+      const double incl = wave_incl_scan(ksum);
+      const double k_tot = read_lane(incl, 63);
       const double u1 = U[2 * h];
       const double u2 = U[2 * h + 1];
-      double xsum = 0.0;
-      for (int p = p0; p < p1; ++p) xsum += kk[p] / k_tot;
-      double cum = wave_excl_scan(xsum);
+      const double thr = u1 * k_tot;
+      double cum = dpp_f64<kDppWaveShr1, 0xf, true>(incl);
       int found = -1;
       for (int p = p0; p < p1; ++p) {
-        cum += kk[p] / k_tot;
-        if (cum > u1) {
+        cum += kk[p];
+        if (cum > thr) {
           found = p;
           break;
         }
@@ -374,12 +381,13 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
 
 // ---------------------------------------------------------------------------
 // Inputs of the one-wavefront kernel.  Per (site, slot) a 64-byte record:
-// the self term A[s, s] - A[s, n], lambda, V_ab, the field shift
-// 0.5 F.d, the hop vector d and the lattice part q_L . (A[n, :] - A[s, :])
-// of the hop's potential difference; per site (phi_L, E); per (site, slot)
-// the target and its packed code (0 without cell-0 rows).
+// the self term A[s, s] - A[s, n], lambda, 1 / (4 lambda), V_ab plus the
+// field shift 0.5 F.d, the hop vector d and the lattice part
+// q_L . (A[n, :] - A[s, :]) of the hop's potential difference; per site
+// (phi_L, E); per (site, slot) the target and its packed code (0 without
+// cell-0 rows).
 struct alignas(16) SlotRec {
-  double tself, lam, vab, shift;
+  double tself, lam, i4l, vs;
   double v[3];
   double dphi;
 };
@@ -402,12 +410,12 @@ struct WaveArgs {
   double* hop_time;
   int32_t* hop_proc;
   int32_t nc, z, mz, field_on;
-  double q0, nu, kT, dt;
+  double q0, nu, nkt, dt;  // nkt = -1 / kT
   int64_t P, n_pairs, max_hops;
 };
 
-// Records of every (site, slot) from the slot tables (the shift with the
-// workgroup kernel's arithmetic: products and sums rounded one by one).
+// Records of every (site, slot) from the slot tables (the workgroup
+// kernel's arithmetic: products and sums rounded one by one).
 __global__ void pack_slot_records(int64_t NM, const double* __restrict__ tself,
                                   const double* __restrict__ dphi,
                                   const double* __restrict__ lam,
@@ -424,11 +432,13 @@ __global__ void pack_slot_records(int64_t NM, const double* __restrict__ tself,
   SlotRec r;
   r.tself = tself[k];
   r.lam = lam[k];
-  r.vab = vab[k];
+  r.i4l = 1.0 / (4 * r.lam);
   r.v[0] = vec[3 * k];
   r.v[1] = vec[3 * k + 1];
   r.v[2] = vec[3 * k + 2];
-  r.shift = field_on ? 0.5 * (f0 * r.v[0] + f1 * r.v[1] + f2 * r.v[2]) : 0.0;
+  const double shift =
+      field_on ? 0.5 * (f0 * r.v[0] + f1 * r.v[1] + f2 * r.v[2]) : 0.0;
+  r.vs = vab[k] + shift;
   r.dphi = dphi[k];
   rec[k] = r;
   nbr2[k] = make_int2(nbr[k], nbr_pk ? int(nbr_pk[k]) : 0);
@@ -478,6 +488,8 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
   const int R = nproc + nc;
   const int ncp = (nc + 1) >> 1;  // carrier pairs of the cache
   const int lane = threadIdx.x;
+  __shared__ double ext[64];  // 2^(j/64)
+  ext[lane] = __builtin_bit_cast(double, kExp2J64[lane]);
   // cache: carriers 2p and 2p + 1 of a row side by side, so that one
   // 16-byte read serves two terms of a row's sum
   double2* G = reinterpret_cast<double2*>(smem);          // ncp x R
@@ -527,8 +539,8 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
   int pc[kNpl], pj[kNpl], px[kNpl];
   uint32_t pxk[kNpl];
   bool pv[kNpl];
-  double pqs[kNpl], pdphi[kNpl], ptq[kNpl], pdE[kNpl], plam[kNpl], pvab[kNpl],
-      psh[kNpl], pvx[kNpl], pvy[kNpl], pvz[kNpl], pst[kNpl];
+  double pqs[kNpl], pdphi[kNpl], ptq[kNpl], pdE[kNpl], plam[kNpl], pi4l[kNpl],
+      pvs[kNpl], pvx[kNpl], pvy[kNpl], pvz[kNpl], pst[kNpl];
   // the site-dependent part of process i's rate: slot j of site s, whose
   // target is n
   auto load_inputs = [&](int i, int s, int j, int n) {
@@ -538,8 +550,8 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     ptq[i] = pqs[i] * r.tself;
     pdE[i] = pn.y - ps.y;
     plam[i] = r.lam;
-    pvab[i] = r.vab;
-    psh[i] = r.shift;
+    pi4l[i] = r.i4l;
+    pvs[i] = r.vs;
     pvx[i] = r.v[0];
     pvy[i] = r.v[1];
     pvz[i] = r.v[2];
@@ -583,8 +595,9 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
       pxk[i] = uint32_t(t.y);
       load_inputs(i, s, pj[i], t.x);
     } else {
-      pdphi[i] = ptq[i] = pdE[i] = pvab[i] = psh[i] = 0.0;
+      pdphi[i] = ptq[i] = pdE[i] = pvs[i] = 0.0;
       plam[i] = 1.0;
+      pi4l[i] = 0.25;
       pvx[i] = pvy[i] = pvz[i] = 0.0;
     }
   }
@@ -659,6 +672,24 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     }
   };
   while (!stop) {
+    // Conditions on the lane are formed again each hop from copies the
+    // compiler cannot move out of the loop: hoisted, each is a 64-bit lane
+    // mask held across the loop, and there are more of them than scalar
+    // registers (they were spilled and reloaded, two instructions each).
+    int lane_h = lane;
+    asm volatile("" : "+v"(lane_h));
+    int pv_h[kNpl], brl_h[kB], bc_h[kB];
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i) {
+      pv_h[i] = pv[i] ? 1 : 0;
+      asm volatile("" : "+v"(pv_h[i]));
+    }
+#pragma unroll
+    for (int k = 0; k < kB; ++k) {
+      brl_h[k] = brl[k];
+      bc_h[k] = bc[k];
+      asm volatile("" : "+v"(brl_h[k]), "+v"(bc_h[k]));
+    }
     if (a.max_hops > 0 && hops >= a.max_hops) {
       code = 0;
       break;
@@ -687,33 +718,29 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
 #pragma clang fp contract(off)
       kk[i] = 0.0;
       gg[i] = 0.0;
-      if (pv[i]) {
+      if (pv_h[i]) {
         const double term01 = pdphi[i] + (pst[i] - psis[pc[i]]);
         const double g0 = 2 * pqs[i] * (term01 + ptq[i]) + pdE[i];
-        const double lam = plam[i];
-        const double gs =
-            (((lam + g0) * (lam + g0) / (4 * lam)) - pvab[i]) - psh[i];
-        kk[i] = a.nu * numpy_e_pow(-gs / a.kT);
+        kk[i] = marcus_rate(g0, plam[i], pi4l[i], pvs[i], a.nkt, a.nu, ext);
         gg[i] = g0;
         ksum += kk[i];
       }
     }
-    // ---- k_tot, the prefix of k / k_tot and the selection
-    const double k_tot = read_lane(wave_incl_scan(ksum), 63);
+    // ---- k_tot and the selection: the first process whose cumulative
+    // rate exceeds u1 k_tot (the workgroup kernel's order of sums)
+    const double incl = wave_incl_scan(ksum);
+    const double k_tot = read_lane(incl, 63);
     const double u1 = read_lane(u1c, hl);
     const double u2 = read_lane(u2c, hl);
     const double lu = read_lane(luc, hl);
-    double xsum = 0.0;
-#pragma unroll
-    for (int i = 0; i < kNpl; ++i)
-      if (pv[i]) xsum += kk[i] / k_tot;
-    double cum = wave_excl_scan(xsum);
+    const double thr = u1 * k_tot;
+    double cum = dpp_f64<kDppWaveShr1, 0xf, true>(incl);
     int found = -1;
 #pragma unroll
     for (int i = 0; i < kNpl; ++i) {
-      if (pv[i] && found < 0) {
-        cum += kk[i] / k_tot;
-        if (cum > u1) found = i;
+      if (pv_h[i] && found < 0) {
+        cum += kk[i];
+        if (cum > thr) found = i;
       }
     }
     const unsigned long long mask = __ballot(found >= 0);
@@ -763,14 +790,14 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     for (int k = 0; k < kB; ++k) {
       gb[k] = 0.0;
       if (k < nb) {
-        const int rl = brl[k] > 0 ? brl[k] : 0;
+        const int rl = brl_h[k] > 0 ? brl_h[k] : 0;
         int x = n;
         uint32_t xk = npk;
         if (rl > 0) {
           x = tt[tsel + rl - 1];
           xk = ttpk[tsel + rl - 1];
         }
-        const int cc = bc[k];
+        const int cc = bc_h[k];
         const int y = cc == m ? n : occ[cc];
         const uint32_t yk = cc == m ? npk : opk[cc];
         gb[k] = entry(x, xk, y, yk);
@@ -778,7 +805,7 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     }
 #pragma unroll
     for (int i = 0; i < kNpl; ++i) {
-      if (pv[i] && pc[i] == m) {
+      if (pv_h[i] && pc[i] == m) {
         const int nn = tt[tsel + pj[i]];
         px[i] = nn;
         pxk[i] = ttpk[tsel + pj[i]];
@@ -799,8 +826,8 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
       end = a.P;
     }
     energy += g_sel;
-    tlog = lane == hl ? t : tlog;
-    plog = lane == hl ? sel : plog;
+    tlog = lane_h == hl ? t : tlog;
+    plog = lane_h == hl ? sel : plog;
     const int64_t row_lo = start;
     int64_t row_hi = start;
     if (end >= start + 1) row_hi = end >= a.P ? a.P : end;
@@ -811,18 +838,18 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     {
       // the mover's displacement and drift, in lane m
 #pragma clang fp contract(off)
-      const bool me = lane == m;
+      const bool me = lane_h == m;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         dsp[k] = me ? dsp[k] + v_sel[k] : dsp[k];
         if (a.field_on) drf[k] = me ? drf[k] + v_sel[k] * k_sel : drf[k];
       }
     }
-    if (lane == 0) {
+    if (lane_h == 0) {
       occ[m] = n;
       opk[m] = npk;
     }
-    if (lane == m) {
+    if (lane_h == m) {
       sx = n;
       sxk = npk;
     }
@@ -830,30 +857,30 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     // rows (which overwrite what the first writes put in them)
 #pragma unroll
     for (int i = 0; i < kNpl; ++i)
-      if (pv[i]) gset(m, lane * kNpl + i, ga[i]);
-    if (lane < nc) gset(m, nproc + lane, gs_site);
+      if (pv_h[i]) gset(m, lane * kNpl + i, ga[i]);
+    if (lane_h < nc) gset(m, nproc + lane, gs_site);
 #pragma unroll
     for (int k = 0; k < kB; ++k) {
-      if (k < nb && brl[k] >= 0) {
-        const int row = brl[k] == 0 ? nproc + m : m * z + brl[k] - 1;
-        gset(bc[k], row, gb[k]);
+      if (k < nb && brl_h[k] >= 0) {
+        const int row = brl_h[k] == 0 ? nproc + m : m * z + brl_h[k] - 1;
+        gset(bc_h[k], row, gb[k]);
       }
     }
-    if (la_on) {
+    if (lane_h < z * z) {
       tt[(m * z + lj) * z + lk] = la.x;
       ttpk[(m * z + lj) * z + lk] = uint32_t(la.y);
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kNpl; ++i)
-      if (pv[i]) pst[i] = row_sum(lane * kNpl + i);
-    if (lane < nc) psis[lane] = row_sum(nproc + lane);
+      if (pv_h[i]) pst[i] = row_sum(lane * kNpl + i);
+    if (lane_h < nc) psis[lane] = row_sum(nproc + lane);
     __syncthreads();
     // ---- path-step outputs, for a hop that ends one or more path steps
     if (row_hi > row_lo) {
       for (int64_t r = row_lo + lane; r < dp_hi; r += 64) a.dpath[r] = g_sel;
       for (int64_t r = row_lo + lane; r < row_hi; r += 64) a.epath[r] = energy;
-      if (lane < nc) {
+      if (lane_h < nc) {
         const int w = 3 * nc;
         for (int64_t r = row_lo; r < row_hi; ++r)
 #pragma unroll
@@ -1366,7 +1393,8 @@ extern "C" int pycd_kmc_create(pycd_ctx* ctx, const pycd_kmc_problem* pb,
     s->wave_npl = (nc * z + 63) / 64;
     // the kernel addresses the table with 32-bit byte offsets
     const double tab_bytes = 8.0 * double(s->compact ? s->geo.nu : N) * N;
-    s->wave_ok = s->wave_smem > 0 && s->wave_smem <= size_t(lds_max) &&
+    // (the kernel's own 512 bytes of 2^(j/64) beside the dynamic part)
+    s->wave_ok = s->wave_smem > 0 && s->wave_smem + 512 <= size_t(lds_max) &&
                  tab_bytes < 4294967296.0;
   }
 #undef S_HIP
@@ -1644,6 +1672,7 @@ int hop_loop(pycd_kmc* s, const double* dU, double* LU, int32_t* status_dev,
   for (int a = 0; a < 3; ++a) d.field[a] = pb.field[a];
   d.nu = pb.nu;
   d.kT = pb.kT;
+  d.nkt = -1.0 / pb.kT;
   d.dt = pb.time_interval;
   d.P = pb.n_path;
   d.occ = s->occ.ptr;
@@ -1683,7 +1712,7 @@ int hop_loop(pycd_kmc* s, const double* dU, double* LU, int32_t* status_dev,
     wa.uq = s->uniform_q ? 1 : 0;
     wa.q0 = s->q0;
     wa.nu = pb.nu;
-    wa.kT = pb.kT;
+    wa.nkt = d.nkt;
     wa.dt = pb.time_interval;
     wa.P = pb.n_path;
     wa.n_pairs = n_pairs;
