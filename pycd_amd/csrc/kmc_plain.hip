@@ -672,24 +672,6 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     }
   };
   while (!stop) {
-    // Conditions on the lane are formed again each hop from copies the
-    // compiler cannot move out of the loop: hoisted, each is a 64-bit lane
-    // mask held across the loop, and there are more of them than scalar
-    // registers (they were spilled and reloaded, two instructions each).
-    int lane_h = lane;
-    asm volatile("" : "+v"(lane_h));
-    int pv_h[kNpl], brl_h[kB], bc_h[kB];
-#pragma unroll
-    for (int i = 0; i < kNpl; ++i) {
-      pv_h[i] = pv[i] ? 1 : 0;
-      asm volatile("" : "+v"(pv_h[i]));
-    }
-#pragma unroll
-    for (int k = 0; k < kB; ++k) {
-      brl_h[k] = brl[k];
-      bc_h[k] = bc[k];
-      asm volatile("" : "+v"(brl_h[k]), "+v"(bc_h[k]));
-    }
     if (a.max_hops > 0 && hops >= a.max_hops) {
       code = 0;
       break;
@@ -718,7 +700,7 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
 #pragma clang fp contract(off)
       kk[i] = 0.0;
       gg[i] = 0.0;
-      if (pv_h[i]) {
+      if (pv[i]) {
         const double term01 = pdphi[i] + (pst[i] - psis[pc[i]]);
         const double g0 = 2 * pqs[i] * (term01 + ptq[i]) + pdE[i];
         kk[i] = marcus_rate(g0, plam[i], pi4l[i], pvs[i], a.nkt, a.nu, ext);
@@ -738,7 +720,7 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     int found = -1;
 #pragma unroll
     for (int i = 0; i < kNpl; ++i) {
-      if (pv_h[i] && found < 0) {
+      if (pv[i] && found < 0) {
         cum += kk[i];
         if (cum > thr) found = i;
       }
@@ -790,14 +772,14 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     for (int k = 0; k < kB; ++k) {
       gb[k] = 0.0;
       if (k < nb) {
-        const int rl = brl_h[k] > 0 ? brl_h[k] : 0;
+        const int rl = brl[k] > 0 ? brl[k] : 0;
         int x = n;
         uint32_t xk = npk;
         if (rl > 0) {
           x = tt[tsel + rl - 1];
           xk = ttpk[tsel + rl - 1];
         }
-        const int cc = bc_h[k];
+        const int cc = bc[k];
         const int y = cc == m ? n : occ[cc];
         const uint32_t yk = cc == m ? npk : opk[cc];
         gb[k] = entry(x, xk, y, yk);
@@ -805,7 +787,7 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     }
 #pragma unroll
     for (int i = 0; i < kNpl; ++i) {
-      if (pv_h[i] && pc[i] == m) {
+      if (pv[i] && pc[i] == m) {
         const int nn = tt[tsel + pj[i]];
         px[i] = nn;
         pxk[i] = ttpk[tsel + pj[i]];
@@ -826,8 +808,8 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
       end = a.P;
     }
     energy += g_sel;
-    tlog = lane_h == hl ? t : tlog;
-    plog = lane_h == hl ? sel : plog;
+    tlog = lane == hl ? t : tlog;
+    plog = lane == hl ? sel : plog;
     const int64_t row_lo = start;
     int64_t row_hi = start;
     if (end >= start + 1) row_hi = end >= a.P ? a.P : end;
@@ -838,18 +820,18 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     {
       // the mover's displacement and drift, in lane m
 #pragma clang fp contract(off)
-      const bool me = lane_h == m;
+      const bool me = lane == m;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         dsp[k] = me ? dsp[k] + v_sel[k] : dsp[k];
         if (a.field_on) drf[k] = me ? drf[k] + v_sel[k] * k_sel : drf[k];
       }
     }
-    if (lane_h == 0) {
+    if (lane == 0) {
       occ[m] = n;
       opk[m] = npk;
     }
-    if (lane_h == m) {
+    if (lane == m) {
       sx = n;
       sxk = npk;
     }
@@ -857,30 +839,30 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     // rows (which overwrite what the first writes put in them)
 #pragma unroll
     for (int i = 0; i < kNpl; ++i)
-      if (pv_h[i]) gset(m, lane * kNpl + i, ga[i]);
-    if (lane_h < nc) gset(m, nproc + lane, gs_site);
+      if (pv[i]) gset(m, lane * kNpl + i, ga[i]);
+    if (lane < nc) gset(m, nproc + lane, gs_site);
 #pragma unroll
     for (int k = 0; k < kB; ++k) {
-      if (k < nb && brl_h[k] >= 0) {
-        const int row = brl_h[k] == 0 ? nproc + m : m * z + brl_h[k] - 1;
-        gset(bc_h[k], row, gb[k]);
+      if (k < nb && brl[k] >= 0) {
+        const int row = brl[k] == 0 ? nproc + m : m * z + brl[k] - 1;
+        gset(bc[k], row, gb[k]);
       }
     }
-    if (lane_h < z * z) {
+    if (la_on) {
       tt[(m * z + lj) * z + lk] = la.x;
       ttpk[(m * z + lj) * z + lk] = uint32_t(la.y);
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kNpl; ++i)
-      if (pv_h[i]) pst[i] = row_sum(lane * kNpl + i);
-    if (lane_h < nc) psis[lane] = row_sum(nproc + lane);
+      if (pv[i]) pst[i] = row_sum(lane * kNpl + i);
+    if (lane < nc) psis[lane] = row_sum(nproc + lane);
     __syncthreads();
     // ---- path-step outputs, for a hop that ends one or more path steps
     if (row_hi > row_lo) {
       for (int64_t r = row_lo + lane; r < dp_hi; r += 64) a.dpath[r] = g_sel;
       for (int64_t r = row_lo + lane; r < row_hi; r += 64) a.epath[r] = energy;
-      if (lane_h < nc) {
+      if (lane < nc) {
         const int w = 3 * nc;
         for (int64_t r = row_lo; r < row_hi; ++r)
 #pragma unroll
