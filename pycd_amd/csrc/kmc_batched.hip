@@ -1081,15 +1081,28 @@ kmc_pool_kernel(BatchArgs a) {
 
 // Squared displacement per trajectory, averaged over its carriers:
 //   msd[t, tau] = mean_c mean_{t0 < P_t - tau} |r_c(t0 + tau) - r_c(t0)|^2
-// One workgroup of 256 per (trajectory, block of kMsdBlock lags).  Each
-// 16-lane row of a wavefront takes one group of kMsdLags lags at a time,
-// its lanes 16 consecutive carriers (so a row's loads are contiguous in the
-// path rows), and walks t0 in chunks of kMsdChunk with the positions
-// r(t0 + tau0 .. t0 + tau0 + kMsdLags + kMsdChunk - 1) of the chunk in
-// registers: a chunk loads 2 kMsdChunk positions and forms
-// kMsdLags kMsdChunk squared distances.  Distances are summed unscaled and
-// scaled by scale^2 once.  The row sums its carriers with DPP and its last
-// lane writes the group's lags.
+// (P_t: the path rows the run wrote).  Two kernels.
+//
+// msd_unit_kernel: one wavefront per (trajectory, carrier) unit.  The
+// carrier's positions pass through LDS (x, y and z arrays) in windows read
+// from global memory once each.  A pass takes kMsdPass lags, lane l the
+// kMsdL consecutive lags tau0 + kMsdL l + i; the lane walks the origins t0
+// of a window with r(t0 + kMsdL l + i), i = 0 .. kMsdL, in a ring of
+// registers, so a step reads the origin (one address for the whole
+// wavefront) and one new position, and forms kMsdL squared distances.  The
+// odd stride kMsdL = 9 puts the 16 lanes of an LDS access on distinct banks.
+// For the first pass the window holds rows [T0, T0 + S + span - 1), which
+// contain the segment's S origins; later passes keep the origins in their
+// own S rows ahead of the window.  Sums over origins, unscaled, per (unit,
+// lag) to scratch.
+//
+// msd_reduce_kernel: one thread per (trajectory, lag) sums the carriers in
+// order and scales by scale^2 / (P_t - tau) / n_c: the same order in every
+// launch, so results repeat bit for bit.
+//
+// Workgroup b runs on XCD b mod 8 (workgroups are dealt to the XCDs in
+// turn); the host deals each trajectory's units to one XCD, so the cache
+// lines of path rows that its carriers share come from HBM once.
 struct MsdTraj {
   int64_t row_off;
   int64_t P;
@@ -1097,167 +1110,150 @@ struct MsdTraj {
   int32_t traj;     // index into the trajectory results (rows written)
   int64_t out_off;
   int64_t M;
+  int64_t sbase;    // scratch offset of carrier 0's M lag sums
 };
 
-constexpr int kMsdLags = 16;    // lags per row
-constexpr int kMsdChunk = 2;    // t0 per chunk
-constexpr int kMsdBlock = 512;  // lags per workgroup (2 passes of 16 rows)
+constexpr int kMsdL = 9;               // lags per lane and pass
+constexpr int kMsdPass = 64 * kMsdL;   // lags per pass
+constexpr int kMsdWin = 1024;          // path rows held in LDS
+constexpr int kMsdXcds = 8;
 
-template <bool kMasked>
-__device__ inline void msd_chunk(const double* wx, const double* wy,
-                                 const double* wz, const double* x0,
-                                 const double* y0, const double* z0,
-                                 int64_t T, int64_t tmax, uint32_t lmask,
-                                 double* acc) {
-#pragma unroll
-  for (int s = 0; s < kMsdChunk; ++s)
-#pragma unroll
-    for (int l = 0; l < kMsdLags; ++l) {
-      const double dx = wx[s + l] - x0[s], dy = wy[s + l] - y0[s],
-                   dz = wz[s + l] - z0[s];
-      if (!kMasked || (((lmask >> l) & 1u) && T + s + l < tmax))
-        acc[l] = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[l])));
-    }
-}
-
-// Sum over the 16 lanes of each row, left in the row's last lane.
-__device__ inline double row_sum16(double v) {
-  v += dpp_f64<kDppRowShr + 1, 0xf, true>(v);
-  v += dpp_f64<kDppRowShr + 2, 0xf, true>(v);
-  v += dpp_f64<kDppRowShr + 4, 0xf, true>(v);
-  v += dpp_f64<kDppRowShr + 8, 0xf, true>(v);
-  return v;
-}
-
-__global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(2)))
-msd_batched_kernel(const double* __restrict__ rows,
-                   const MsdTraj* __restrict__ tr,
-                   const TrajOut* __restrict__ outs, int n_traj, int nlb_max,
-                   double scale, double* __restrict__ out) {
-  constexpr int L = kMsdLags, C = kMsdChunk;
-  constexpr int kGroups = kMsdBlock / L;  // lag groups per workgroup
-  const int t = blockIdx.x / nlb_max;
-  const int lb = blockIdx.x - t * nlb_max;
-  if (t >= n_traj) return;
-  const MsdTraj d = tr[t];
+__global__ void __launch_bounds__(64)
+msd_unit_kernel(const double* __restrict__ rows,
+                const MsdTraj* __restrict__ tr,
+                const TrajOut* __restrict__ outs,
+                const int32_t* __restrict__ units,
+                double* __restrict__ scratch) {
+  __shared__ double X[kMsdWin], Y[kMsdWin], Z[kMsdWin];
+  const int32_t u = units[blockIdx.x];
+  if (u < 0) return;  // padding of an XCD's list
+  const MsdTraj d = tr[u >> 6];
+  const int c = u & 63;
+  const int lane = threadIdx.x;
   // only the rows written by the run: [0, start) (all P when the path filled)
   const int64_t written = outs[d.traj].start;
   const int64_t P = written < 1 ? 1 : (written < d.P ? written : d.P);
-  const int64_t blk0 = int64_t(lb) * kMsdBlock;
-  if (blk0 >= d.M) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int cl = lane & 15, rw = lane >> 4;
-  const int w = 3 * d.nc;
+  const int64_t w3 = 3 * int64_t(d.nc);
+  const gptr<double> src = (gptr<double>)(rows + d.row_off + 3 * c);
+  double* out = scratch + d.sbase + int64_t(c) * d.M;
+  for (int64_t pass0 = 0; pass0 < d.M; pass0 += kMsdPass) {
+    const int span = int(min<int64_t>(kMsdPass, d.M - pass0));
+    const int64_t lo = pass0 + int64_t(kMsdL) * lane;  // this lane's lags
+    double acc[kMsdL];
+#pragma unroll
+    for (int i = 0; i < kMsdL; ++i) acc[i] = 0.0;
+    // origins per segment, and where the window rows sit in LDS
+    const bool first = pass0 == 0;
+    const int S = first ? kMsdWin - span + 1 : (kMsdWin - span + 1) / 2;
+    const int wofs = first ? 0 : S;  // LDS row of window row T0 + pass0
+    // origins t0 < P - pass0 have a lag of this pass inside the rows
+    const int64_t t_end = P - pass0;
+    for (int64_t T0 = 0; T0 < t_end; T0 += S) {
+      const int64_t T1 = min<int64_t>(T0 + S, t_end);
+      // ---- stage the window (and, after the first pass, the origins)
+      const int64_t wlo = T0 + pass0;
+      const int nw = int(min<int64_t>(P, T1 - 1 + pass0 + span) - wlo);
+      __syncthreads();
+      for (int r = lane; r < nw; r += 64) {
+        const int64_t o = (wlo + r) * w3;
+        X[wofs + r] = src[o];
+        Y[wofs + r] = src[o + 1];
+        Z[wofs + r] = src[o + 2];
+      }
+      if (!first)
+        for (int r = lane; r < T1 - T0; r += 64) {
+          const int64_t o = (T0 + r) * w3;
+          X[r] = src[o];
+          Y[r] = src[o + 1];
+          Z[r] = src[o + 2];
+        }
+      __syncthreads();
+      // every lane walks the same origins (its ring slots line up with the
+      // others'); a lag whose row lies past the written rows, or past M,
+      // adds nothing (lags past M are dropped at the end)
+      const int rel = int(lo - pass0) + wofs;  // LDS row of r(T0 + lo)
+      double wx[kMsdL + 1], wy[kMsdL + 1], wz[kMsdL + 1];
+      auto ld = [&](int row, double& x, double& y, double& z) {
+        const int rr = row < kMsdWin ? row : kMsdWin - 1;
+        x = X[rr];
+        y = Y[rr];
+        z = Z[rr];
+      };
+#pragma unroll
+      for (int i = 0; i < kMsdL; ++i) ld(rel + i, wx[i], wy[i], wz[i]);
+      // the highest first lag among lanes with a lag inside the pass and the
+      // written rows: below fast_end every such lane has all its rows
+      const int64_t reach = min<int64_t>(span, P - pass0);
+      const int64_t lo_top = pass0 + kMsdL * ((reach - 1) / kMsdL);
+      const int64_t fast_end = min<int64_t>(T1, P - (lo_top + kMsdL - 1));
+      constexpr int kR = kMsdL + 1;  // ring slots; a turn is kR origins
+      int64_t t0 = T0;
+      // Whole turns: at step j of a turn slot (p % kR) holds
+      // r(t0 + lo + p) for p = j .. j + kMsdL - 1; the step loads position
+      // j + kMsdL into the slot position j - 1 left
+      for (; t0 + kR <= fast_end; t0 += kR) {
+#pragma unroll
+        for (int j = 0; j < kR; ++j) {
+          const int oi = int(t0 - T0) + j;
+          ld(rel + oi + kMsdL, wx[(j + kMsdL) % kR], wy[(j + kMsdL) % kR],
+             wz[(j + kMsdL) % kR]);
+          const double ox = X[oi], oy = Y[oi], oz = Z[oi];
+#pragma unroll
+          for (int i = 0; i < kMsdL; ++i) {
+            const int q = (j + i) % kR;
+            const double dx = wx[q] - ox, dy = wy[q] - oy, dz = wz[q] - oz;
+            acc[i] = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[i])));
+          }
+        }
+      }
+      // the rest: lags checked against the written rows, origins against
+      // the segment's end
+      for (; t0 < T1; t0 += kR) {
+#pragma unroll
+        for (int j = 0; j < kR; ++j) {
+          if (t0 + j >= T1) break;
+          const int oi = int(t0 - T0) + j;
+          ld(rel + oi + kMsdL, wx[(j + kMsdL) % kR], wy[(j + kMsdL) % kR],
+             wz[(j + kMsdL) % kR]);
+          const double ox = X[oi], oy = Y[oi], oz = Z[oi];
+          const int64_t nv = P - (t0 + j + lo);  // lags i < nv have rows
+#pragma unroll
+          for (int i = 0; i < kMsdL; ++i) {
+            const int q = (j + i) % kR;
+            const double dx = wx[q] - ox, dy = wy[q] - oy, dz = wz[q] - oz;
+            const double a = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[i])));
+            acc[i] = i < nv ? a : acc[i];
+          }
+        }
+      }
+    }
+    if (lo < d.M) {
+#pragma unroll
+      for (int i = 0; i < kMsdL; ++i)
+        if (lo + i < d.M) out[lo + i] = acc[i];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+msd_reduce_kernel(const MsdTraj* __restrict__ tr, int64_t n_traj,
+                  const TrajOut* __restrict__ outs,
+                  const double* __restrict__ scratch, double scale,
+                  double* __restrict__ out) {
+  const int64_t k = blockIdx.x;
+  if (k >= n_traj) return;
+  const MsdTraj d = tr[k];
+  const int64_t written = outs[d.traj].start;
+  const int64_t P = written < 1 ? 1 : (written < d.P ? written : d.P);
   const double s2 = scale * scale;
-  // kGroups groups of L lags: 16 rows (4 per wavefront) per pass
-  for (int pass = 0; pass < kGroups / 16; ++pass) {
-    const int grp = pass * 16 + wave * 4 + rw;
-    const int64_t tau0 = blk0 + int64_t(L) * grp;
-    if (blk0 + int64_t(L) * (pass * 16 + wave * 4) >= d.M) break;  // wavefront
-    uint32_t lmask = 0;  // lags of this row inside [1, M)
-#pragma unroll
-    for (int l = 0; l < L; ++l)
-      if (tau0 + l >= 1 && tau0 + l < d.M) lmask |= 1u << l;
-    const int64_t tmax = P - tau0;  // t0 < tmax for lag tau0
-    double acc[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) acc[l] = 0.0;
-    if (lmask && tmax > 0) {
-      // path rows of this trajectory: a scalar base and 32-bit per-lane
-      // byte offsets (a trajectory's rows span less than 4 GiB)
-      const gptr<char> row0 = (gptr<char>)(rows + d.row_off);
-      const uint32_t stride = uint32_t(w) * 8u;
-      const int64_t last = P - 1;
-      for (int c0 = 0; c0 < d.nc; c0 += 16) {
-        // a block of nb < 16 carriers is repeated over ph = 16 / nb groups
-        // of lanes, each taking a contiguous share of the origins
-        const int nb = min(16, d.nc - c0), ph = 16 / nb;
-        if (cl >= nb * ph) continue;
-        const int c = c0 + cl % nb, phase = cl / nb;
-        const int64_t seg = ((tmax + ph - 1) / ph + C - 1) / C * C;
-        const int64_t T0 = min(tmax, phase * seg), T1 = min(tmax, T0 + seg);
-        const uint32_t cb = uint32_t(c) * 24u;
-        auto load = [&](uint32_t off, double& x, double& y, double& z) {
-          const gptr<double> q = (gptr<double>)(row0 + off);
-          x = q[0];
-          y = q[1];
-          z = q[2];
-        };
-        // position at path step tt, clamped into the written rows
-        auto load_at = [&](int64_t tt, double& x, double& y, double& z) {
-          load(cb + uint32_t(tt < last ? tt : last) * stride, x, y, z);
-        };
-        double wx[L + C], wy[L + C], wz[L + C];
-#pragma unroll
-        for (int k = 0; k < L; ++k)
-          load_at(tau0 + T0 + k, wx[k], wy[k], wz[k]);
-        int64_t T = T0;
-        // Whole turns of the window ring: kU chunks advance it by its
-        // length, so slot (j C + i) % kW of chunk j holds r(tau0 + T + j C
-        // + i) and no position is moved between registers; every load and
-        // every distance of a turn is in range.
-        constexpr int kW = L + C, kU = kW / C;
-        static_assert(kW % C == 0, "the ring turns in whole chunks");
-        if (lmask == (1u << L) - 1u) {
-          uint32_t ow = cb + uint32_t(tau0 + T0 + L) * stride,
-                   oo = cb + uint32_t(T0) * stride;
-          for (; T + kU * C + L - 1 < tmax && T + kU * C <= T1; T += kU * C) {
-#pragma unroll
-            for (int j = 0; j < kU; ++j) {
-              double x0[C], y0[C], z0[C];
-#pragma unroll
-              for (int k = 0; k < C; ++k) {
-                const int sl = (j * C + L + k) % kW;
-                load(ow, wx[sl], wy[sl], wz[sl]);
-                load(oo, x0[k], y0[k], z0[k]);
-                ow += stride;
-                oo += stride;
-              }
-#pragma unroll
-              for (int s2i = 0; s2i < C; ++s2i)
-#pragma unroll
-                for (int l = 0; l < L; ++l) {
-                  const int sl = (j * C + s2i + l) % kW;
-                  const double dx = wx[sl] - x0[s2i], dy = wy[sl] - y0[s2i],
-                               dz = wz[sl] - z0[s2i];
-                  acc[l] = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[l])));
-                }
-            }
-          }
-        }
-        // the rest chunk by chunk, the window shifted through registers
-        for (; T < T1; T += C) {
-          double x0[C], y0[C], z0[C];
-#pragma unroll
-          for (int k = 0; k < C; ++k) {
-            load_at(tau0 + T + L + k, wx[L + k], wy[L + k], wz[L + k]);
-            load_at(T + k, x0[k], y0[k], z0[k]);
-          }
-          msd_chunk<true>(wx, wy, wz, x0, y0, z0, T, tmax, lmask, acc);
-#pragma unroll
-          for (int k = 0; k < L; ++k) {
-            wx[k] = wx[k + C];
-            wy[k] = wy[k + C];
-            wz[k] = wz[k + C];
-          }
-        }
-      }
+  for (int64_t tau = threadIdx.x; tau < d.M; tau += blockDim.x) {
+    double v = 0.0;
+    if (tau >= 1 && tau < P) {
+      double acc = 0.0;
+      for (int c = 0; c < d.nc; ++c) acc += scratch[d.sbase + c * d.M + tau];
+      v = acc * s2 / double(P - tau) / double(d.nc);
     }
-    // every lane of the wavefront takes part in the row sums
-#pragma unroll
-    for (int l = 0; l < L; ++l) acc[l] = row_sum16(acc[l]);
-    if (cl == 15) {
-#pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const int64_t tau = tau0 + l;
-        if (tau >= d.M) break;
-        double v = 0.0;
-        if (tau >= 1 && tau < P)
-          v = acc[l] * s2 / double(P - tau) / double(d.nc);
-        out[d.out_off + tau] = v;
-      }
-    }
+    out[d.out_off + tau] = v;
   }
 }
 
@@ -1372,12 +1368,16 @@ struct pycd_batch {
   std::vector<char> slot_busy;
   std::vector<MsdTraj> msd_host;
   DevBuf<MsdTraj> d_msdtr;
+  // MSD launch plan: (trajectory, carrier) units in workgroup order and
+  // the per-unit lag sums
+  DevBuf<int32_t> d_msd_units;
+  DevBuf<double> d_msd_scratch;
+  int64_t msd_units = 0;
   std::vector<TrajDev> trajs;
   std::vector<int64_t> msd_off;
   int64_t n_traj = 0, rows_total = 0, msd_total = 0;
   int nc_max = 0, z_max = 0;
   int sub_max = kMaxSubSlots;
-  int nlb_max = 0;
   double msd_scale = 1.0;
   size_t smem = 0;
   int grid = 0;

@@ -243,3 +243,34 @@ def test_fp64_batch_monoclinic_matches_oracle_per_trajectory():
         assert abs(res["energy"][k] - ora["energy_now"]) < 1e-9 * scale, k
         assert res["start"][k] == ora["start"]
     bt.close()
+
+
+def test_msd_lag_passes_and_many_carriers():
+    """The MSD kernels take lags in passes of 576 and one wavefront per
+    (trajectory, carrier): lags reaching into the second and third pass,
+    and 64, 33 and 2 electrons, against the oracle's squared displacements
+    (each path filled: 2001 rows)."""
+    fams, runs = _families([4], t_final=2e-8)
+    specs = [(0, 64, 0.0, 300.0), (0, 2, 1e6, 350.0), (0, 33, 0.0, 280.0)]
+    seed = 23
+    trajs, occs, words = _mixed_batch(fams, runs, specs, seed, 60000)
+    for t, (f, nc, fv, T) in zip(trajs, specs):
+        rate = B.expected_rate_per_carrier(fams[f], t["kT"], t["field"]) * nc
+        t["time_interval"] = 6000.0 / rate / 2000
+        t["n_path"] = 2001
+    lags = [1300, 700, 577]
+    scale = 1 / constants.ANG2BOHR
+    bt = B.KmcBatch(fams, "f64", "f32")
+    bt.setup(trajs, occs, words, msd_lags=lags, msd_scale=scale,
+             waves_per_cu=1)
+    bt.run(use_graph=False)
+    res = bt.results()
+    got = bt.msd()
+    for k, (f, nc, fv, T) in enumerate(specs):
+        P = int(res["start"][k])
+        assert P == 2001, (k, P, res["hops"][k])
+        pos = bt.rows(k)[:P].reshape(P, nc, 3) * scale
+        ref = omsd.squared_displacement(pos, 1, P, lags[k])[0].mean(axis=1)
+        assert got[k][0] == 0.0
+        assert np.allclose(got[k], ref, rtol=1e-11, atol=1e-300), k
+    bt.close()

@@ -57,3 +57,34 @@ def test_min_image_vectors(hematite):
                        rtol=0, atol=1e-12)
     f = (gpu - ref).reshape(-1, 3) @ np.linalg.inv(cell)
     assert np.max(np.abs(f - np.round(f))) < 1e-9
+
+
+def test_neighbor_list_file_round_trip(tmp_path, hematite):
+    """Neighbors.generate_neighbor_list writes hop_neighbor_list.npy; loaded
+    the way the reference's drivers load it (np.load with allow_pickle,
+    then [()], PyCD/material_run.py:68-69, a file this package wrote), it
+    holds the oracle's lists, and a Run built from the loaded lists replays
+    the shipped trajectory's first hops."""
+    from pycd_amd.core import Run, System
+    from tests.kmc_helpers import run_gpu
+    ex = hematite
+    ex.neighbors.generate_neighbor_list(tmp_path, [3, 3, 3])
+    path = tmp_path / "hop_neighbor_list.npy"
+    assert (tmp_path / "neighbor_list.log").exists()
+    loaded = np.load(path, allow_pickle=True)[()]
+    ref = hop_neighbor_list(ex.material, ex.neighbors)
+    _compare_lists(loaded, ref, ex.neighbors.simulation_cell_matrix)
+    system = System(ex.material, ex.neighbors, loaded, ex.pairwise,
+                    ex.cfg["alpha"], ex.cfg["r_cut"], ex.cfg["k_cut"],
+                    ex.cfg["precision_parameters"], [], [])
+    run = Run(system, ex.table, ex.sim["temp"], ex.sim["ion_charge_type"],
+              ex.sim["species_charge_type"], ex.sim["n_traj"],
+              ex.sim["t_final"], ex.sim["time_interval"],
+              np.array(ex.sim["species_count"]), ex.sim["initial_occupancy"],
+              ex.sim["relative_energies"], ex.sim["external_field"],
+              ex.sim["doping"])
+    res, _ = run_gpu(run, ex.table, 2, 1, max_hops=2000, record=False)
+    td = ex.time_data[:res["hops"] + 1]
+    assert res["hops"] == 2000
+    rel = np.abs(res["time_data"][1:] - td[1:]) / td[1:]
+    assert rel.max() < 1e-12
