@@ -160,13 +160,21 @@ table_mfma_f32(const float* __restrict__ Y, int n, int Kp,
   count_pairs(cnt, pairs);
 }
 
+// K is staged in slices of 64 (kBSlice): two 32-deep MFMA steps per slice
+// between barriers, and a slice's global loads (in registers while the
+// previous slice is multiplied) have twice the matrix work to arrive in
+// than 32-deep slices gave them.  Padded rows of 72 bf16 (36 dwords) put
+// the 8 rows one 16-byte LDS read serves on distinct banks.
+constexpr int kBSlice = 64;
+
 __global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(2, 2)))
 table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
                   const __bf16* __restrict__ Ylo, int n, int Kp,
                   const double* __restrict__ pos, MicGeometry g,
                   EwaldScalars es, double* __restrict__ A,
                   unsigned long long* __restrict__ pairs) {
-  constexpr int kRow = kMSlice + 8;  // padded row (bf16 elements)
+  constexpr int kRow = kBSlice + 8;  // padded row (bf16 elements)
   __shared__ __attribute__((aligned(16))) __bf16 Ah[kMTile * kRow];
   __shared__ __attribute__((aligned(16))) __bf16 Al[kMTile * kRow];
   __shared__ __attribute__((aligned(16))) __bf16 Bh[kMTile * kRow];
@@ -182,18 +190,17 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  // loader: 128 rows x 32 bf16 = 256 threads x 16 bf16 (two 16-byte chunks)
-  const int lrow = t >> 1, lk = (t & 1) * 16;
+  // loader: 128 rows x 64 bf16 = 256 threads x 32 bf16 (four 16-byte
+  // chunks) per array
+  const int lrow = t >> 1, lk = (t & 1) * 32;
   const bool a_ok = i0 + lrow < n, b_ok = j0 + lrow < n;
   const int64_t aoff = int64_t(i0 + lrow) * Kp + lk;
   const int64_t boff = int64_t(j0 + lrow) * Kp + lk;
   const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
-  // software pipeline: slice k0 + kMSlice is loaded into registers while the
-  // matrix cores work on slice k0 from LDS
-  uint4 ah[2], al[2], bh[2], bl[2];
+  uint4 ah[4], al[4], bh[4], bl[4];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 4; ++q) {
       ah[q] = a_ok ? load16_global(Yhi + aoff + k0 + 8 * q) : zero;
       al[q] = a_ok ? load16_global(Ylo + aoff + k0 + 8 * q) : zero;
       bh[q] = b_ok ? load16_global(Yhi + boff + k0 + 8 * q) : zero;
@@ -202,7 +209,7 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
   };
   auto sstore = [&]() {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 4; ++q) {
       *reinterpret_cast<uint4*>(Ah + lrow * kRow + lk + 8 * q) = ah[q];
       *reinterpret_cast<uint4*>(Al + lrow * kRow + lk + 8 * q) = al[q];
       *reinterpret_cast<uint4*>(Bh + lrow * kRow + lk + 8 * q) = bh[q];
@@ -212,31 +219,36 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
   gload(0);
   sstore();
   __syncthreads();
-  for (int k0 = 0; k0 < Kp; k0 += kMSlice) {
-    const bool more = k0 + kMSlice < Kp;
-    if (more) gload(k0 + kMSlice);
-    v8bf a_h[4], a_l[4], b_h[4], b_l[4];
-    const int kof = (lane >> 4) * 8;
+  for (int k0 = 0; k0 < Kp; k0 += kBSlice) {
+    const bool more = k0 + kBSlice < Kp;
+    if (more) gload(k0 + kBSlice);
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-      const int ra = (wr + 16 * x + (lane & 15)) * kRow + kof;
-      const int rb = (wc + 16 * x + (lane & 15)) * kRow + kof;
-      a_h[x] = *reinterpret_cast<const v8bf*>(Ah + ra);
-      a_l[x] = *reinterpret_cast<const v8bf*>(Al + ra);
-      b_h[x] = *reinterpret_cast<const v8bf*>(Bh + rb);
-      b_l[x] = *reinterpret_cast<const v8bf*>(Bl + rb);
-    }
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
+    for (int h = 0; h < kBSlice / 32; ++h) {
+      // this wavefront's column blocks once, then one row block at a time
+      const int kof = (lane >> 4) * 8 + 32 * h;
+      v8bf b_h[4], b_l[4];
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
-        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l[x], b_h[y],
-                                                            acc[x][y], 0, 0, 0);
-        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[x], b_l[y],
-                                                            acc[x][y], 0, 0, 0);
-        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[x], b_h[y],
-                                                            acc[x][y], 0, 0, 0);
+        const int rb = (wc + 16 * y + (lane & 15)) * kRow + kof;
+        b_h[y] = *reinterpret_cast<const v8bf*>(Bh + rb);
+        b_l[y] = *reinterpret_cast<const v8bf*>(Bl + rb);
       }
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const int ra = (wr + 16 * x + (lane & 15)) * kRow + kof;
+        const v8bf a_h = *reinterpret_cast<const v8bf*>(Ah + ra);
+        const v8bf a_l = *reinterpret_cast<const v8bf*>(Al + ra);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              a_l, b_h[y], acc[x][y], 0, 0, 0);
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              a_h, b_l[y], acc[x][y], 0, 0, 0);
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              a_h, b_h[y], acc[x][y], 0, 0, 0);
+        }
+      }
+    }
     __syncthreads();  // every wavefront is done with this slice
     if (more) sstore();
     __syncthreads();
@@ -265,7 +277,9 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
                       double* out, unsigned long long* pairs,
                       hipEvent_t after_sf) {
   hipStream_t s = ctx->stream;
-  const int Kp = std::max(kMSlice, ((2 * K + kMSlice - 1) / kMSlice) * kMSlice);
+  // K padded to whole slices of the mode's kernel (zero columns add nothing)
+  const int slice = precision == PYCD_PREC_F32 ? kMSlice : kBSlice;
+  const int Kp = std::max(slice, ((2 * K + slice - 1) / slice) * slice);
   const size_t elems = size_t(n) * Kp;
   DevBuf<float> y32;
   DevBuf<__bf16> yhi, ylo;
