@@ -286,15 +286,23 @@ def run_precision(args, log):
     |k| < 2 pi 10 / L_a, table in FP64 / FP32-MFMA / BF16x3-MFMA; per-entry
     and total-energy error against FP64; then 1e6 hops with 32 electrons on
     each table (plain path), compared with the FP64-table run."""
-    from pycd_amd import workloads, _native, rng, constants
+    from pycd_amd import workloads, _native, rng, constants, tuning
     from pycd_amd.rng import seeded_starts
     n = args.precision_edge
     w = workloads.corundum(n, m_kcut=n, n_carriers=32, t_final=1e-8)
     _KEEP["precision"] = w
     pos = w.neighbors.bulk_sites.cell_coordinates
-    out = {"n_sites": int(w.n_sites)}
+    out = {"n_sites": int(w.n_sites), "autotune": {}}
     tables = {}
+    targs = (pos, w.neighbors.simulation_cell_matrix,
+             w.system.translational_matrix, w.system.pbc, w.system.alpha,
+             w.system.r_cut, w.system.k_cut, 1.0)
     for mode in ("f64", "f32", "bf16x3"):
+        # the tile variant for this lattice, by the autotuner
+        v, times = tuning.autotune_table(*targs, precision=mode, repeats=1,
+                                         log=log)
+        out["autotune"][mode] = {"variant": v,
+                                 "ms": {str(k): t for k, t in times.items()}}
         best = None
         for rep_i in range(2):
             tab, rep = _native.pair_table(

@@ -103,6 +103,19 @@ int pycd_pair_table(pycd_ctx* ctx, const double* pos, int64_t n,
                     double k_cut, double eps_r, int32_t precision,
                     double* table_out, pycd_ewald_report* report);
 
+/* Tile variant of the table kernels for one precision, kept in the context
+ * for its later pycd_pair_table calls.  No counterpart in the reference (it
+ * has no kernels): the autotuner (pycd_amd/tuning.py autotune_table) times
+ * the variants and sets the fastest.  Every variant gives the same table
+ * bit for bit.
+ *   PYCD_PREC_F64     1 = 64 x 64 tiles, 2 = 128 x 128 tiles
+ *   PYCD_PREC_BF16X3  1 = K staged in slices of 32, 2 = slices of 64
+ *   PYCD_PREC_F32     1 (the only one)
+ *   0 = the default (FP64: 128 x 128 from 2048 sites on; BF16x3: 64). */
+int pycd_set_table_variant(pycd_ctx* ctx, int32_t precision, int32_t variant);
+/* The variant set for `precision` (0 = default), or -1 for a bad call. */
+int pycd_table_variant(const pycd_ctx* ctx, int32_t precision);
+
 /* ------------------------------------------------------------ neighbours -- */
 /* N x N x 3 minimum-image vectors pos[j] - pos[i]
  * (Neighbors.get_pairwise_min_image_vector_data, PyCD/core.py:571-594).

@@ -92,6 +92,8 @@ _SIGNATURES = {
     "pycd_kmc_begin": (C.c_int, [_P, _P, C.c_double]),
     "pycd_kmc_set_sites": (C.c_int, [_P, _P, _P]),
     "pycd_kmc_table_rows": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "pycd_set_table_variant": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "pycd_table_variant": (C.c_int, [_P, C.c_int32]),
     "pycd_kmc_set_variant": (C.c_int, [_P, C.c_int32]),
     "pycd_kmc_variant": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "pycd_kmc_advance": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P,
@@ -219,6 +221,25 @@ def pair_table(pos, cell, trans, pbc, alpha, r_cut, k_cut, eps_r,
               "ms_real": rep.ms_real, "ms_recip": rep.ms_recip,
               "ms_total": rep.ms_total}
     return out, report
+
+
+def set_table_variant(precision, variant):
+    """Tile variant of the table kernels for `precision` in this process's
+    context (0 = default; FP64: 1 = 64 x 64 tiles, 2 = 128 x 128; BF16x3:
+    1 = K slices of 32, 2 = of 64; FP32: 1).  Every variant gives the same
+    table bit for bit."""
+    ctx = context()
+    _check(ctx, lib().pycd_set_table_variant(ctx, PREC[precision],
+                                             int(variant)),
+           "pycd_set_table_variant")
+
+
+def table_variant(precision):
+    """The table kernel variant set for `precision` (0 = default)."""
+    return int(lib().pycd_table_variant(context(), PREC[precision]))
+
+
+TABLE_VARIANTS = {"f64": (1, 2), "f32": (1,), "bf16x3": (1, 2)}
 
 
 class DeviceArray:

@@ -21,6 +21,8 @@ struct pycd_ctx {
   std::vector<uint32_t> simd_keys;  // hardware keys of the SIMDs (probed once)
   // device address of the static scratch of pycd_kmc_advance_static
   double* static_scratch = nullptr;
+  // table kernel variant per PYCD_PREC_* (0: the default choice)
+  int table_variant[3] = {0, 0, 0};
 };
 
 namespace pycd {

@@ -79,3 +79,24 @@ extern "C" int pycd_memcpy(pycd_ctx* ctx, void* dst, const void* src,
   PYCD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return PYCD_OK;
 }
+
+extern "C" int pycd_set_table_variant(pycd_ctx* ctx, int precision,
+                                      int variant) {
+  if (!ctx) return PYCD_ERR_ARG;
+  if (precision < PYCD_PREC_F64 || precision > PYCD_PREC_BF16X3)
+    return pycd::fail(ctx, PYCD_ERR_ARG, "unknown precision %d", precision);
+  // FP64: 1 = 64 x 64 tiles, 2 = 128 x 128; BF16x3: 1 = K slices of 32,
+  // 2 = slices of 64; FP32: 1 only; 0 = default
+  const int hi = precision == PYCD_PREC_F32 ? 1 : 2;
+  if (variant < 0 || variant > hi)
+    return pycd::fail(ctx, PYCD_ERR_ARG, "variant %d outside 0..%d", variant,
+                      hi);
+  ctx->table_variant[precision] = variant;
+  return PYCD_OK;
+}
+
+extern "C" int pycd_table_variant(const pycd_ctx* ctx, int precision) {
+  if (!ctx || precision < PYCD_PREC_F64 || precision > PYCD_PREC_BF16X3)
+    return -1;
+  return ctx->table_variant[precision];
+}

@@ -40,16 +40,21 @@ structure_factor_f64(const double* __restrict__ pos, int n,
 }
 
 // ------------------------------------------------- FP64 contraction + epilogue
-// One 64 x 64 tile of A per 256-thread workgroup; thread (tx, ty) owns rows
-// ty + 16 r and columns tx + 16 c (r, c < 4).  K is staged through LDS in
-// slices of 16.
-constexpr int kTile = 64;
+// One (16 kRt) x (16 kRt) tile of A per 256-thread workgroup; thread
+// (tx, ty) owns rows ty + 16 r and columns tx + 16 c (r, c < kRt).  K is
+// staged through LDS in slices of 16.  A thread reads 2 kRt LDS values per
+// k for kRt^2 FMAs: kRt = 4 (64 x 64 tiles) needs 4 bytes of LDS per FMA,
+// twice what the LDS can feed the FP64 units; kRt = 8 (128 x 128) halves
+// that at a quarter of the workgroups.  Every entry's sum over k runs in
+// the same order in both, so they give the same table bit for bit.
 constexpr int kSlice = 16;
 
+template <int kRt>
 __global__ void __launch_bounds__(256)
 table_f64(const double* __restrict__ Y, int n, int Kp,
           const double* __restrict__ pos, MicGeometry g, EwaldScalars es,
           double* __restrict__ A, unsigned long long* __restrict__ pairs) {
+  constexpr int kTile = 16 * kRt;
   __shared__ double As[kSlice][kTile + 2];
   __shared__ double Bs[kSlice][kTile + 2];
   const int t = threadIdx.x;
@@ -59,61 +64,79 @@ table_f64(const double* __restrict__ Y, int n, int Kp,
   const int i0 = bi * kTile, j0 = bj * kTile;
   const bool mirror = bi != bj;
 
-  double acc[4][4];
+  double acc[kRt][kRt];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int r = 0; r < kRt; ++r)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+    for (int c = 0; c < kRt; ++c) acc[r][c] = 0.0;
 
-  // loader mapping: 64 rows x 16 k = 256 threads x 4 doubles
-  const int lrow = t >> 2;
-  const int lk = (t & 3) * 4;
-  const bool arow_ok = (i0 + lrow) < n;
-  const bool brow_ok = (j0 + lrow) < n;
-  const double* ya = Y + int64_t(i0 + lrow) * Kp + lk;
-  const double* yb = Y + int64_t(j0 + lrow) * Kp + lk;
+  // loader: kTile rows x 16 k = 256 threads x (kTile / 16) doubles per
+  // array, as double2 pairs of consecutive k
+  constexpr int kPer = kTile / 16;          // doubles per thread and array
+  constexpr int kRowsPerPass = 256 * 4 / 16;  // rows per pass of 4 doubles
+  auto load_rows = [&](int k0, double (&av)[kPer], double (&bv)[kPer]) {
+#pragma unroll
+    for (int q = 0; q < kPer / 4; ++q) {
+      const int row = (t >> 2) + kRowsPerPass * q;
+      const int lk = (t & 3) * 4;
+      double2 a01 = make_double2(0.0, 0.0), a23 = a01, b01 = a01, b23 = a01;
+      if (i0 + row < n) {
+        const double* ya = Y + int64_t(i0 + row) * Kp + k0 + lk;
+        a01 = *reinterpret_cast<const double2*>(ya);
+        a23 = *reinterpret_cast<const double2*>(ya + 2);
+      }
+      if (j0 + row < n) {
+        const double* yb = Y + int64_t(j0 + row) * Kp + k0 + lk;
+        b01 = *reinterpret_cast<const double2*>(yb);
+        b23 = *reinterpret_cast<const double2*>(yb + 2);
+      }
+      av[4 * q] = a01.x;
+      av[4 * q + 1] = a01.y;
+      av[4 * q + 2] = a23.x;
+      av[4 * q + 3] = a23.y;
+      bv[4 * q] = b01.x;
+      bv[4 * q + 1] = b01.y;
+      bv[4 * q + 2] = b23.x;
+      bv[4 * q + 3] = b23.y;
+    }
+  };
 
   for (int k0 = 0; k0 < Kp; k0 += kSlice) {
-    double2 a01 = make_double2(0.0, 0.0), a23 = a01, b01 = a01, b23 = a01;
-    if (arow_ok) {
-      a01 = *reinterpret_cast<const double2*>(ya + k0);
-      a23 = *reinterpret_cast<const double2*>(ya + k0 + 2);
-    }
-    if (brow_ok) {
-      b01 = *reinterpret_cast<const double2*>(yb + k0);
-      b23 = *reinterpret_cast<const double2*>(yb + k0 + 2);
-    }
+    double av[kPer], bv[kPer];
+    load_rows(k0, av, bv);
     __syncthreads();
-    As[lk + 0][lrow] = a01.x;
-    As[lk + 1][lrow] = a01.y;
-    As[lk + 2][lrow] = a23.x;
-    As[lk + 3][lrow] = a23.y;
-    Bs[lk + 0][lrow] = b01.x;
-    Bs[lk + 1][lrow] = b01.y;
-    Bs[lk + 2][lrow] = b23.x;
-    Bs[lk + 3][lrow] = b23.y;
+#pragma unroll
+    for (int q = 0; q < kPer / 4; ++q) {
+      const int row = (t >> 2) + kRowsPerPass * q;
+      const int lk = (t & 3) * 4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        As[lk + e][row] = av[4 * q + e];
+        Bs[lk + e][row] = bv[4 * q + e];
+      }
+    }
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < kSlice; ++kk) {
-      double a[4], b[4];
+      double a[kRt], b[kRt];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[kk][ty + 16 * r];
+      for (int r = 0; r < kRt; ++r) a[r] = As[kk][ty + 16 * r];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) b[c] = Bs[kk][tx + 16 * c];
+      for (int c = 0; c < kRt; ++c) b[c] = Bs[kk][tx + 16 * c];
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+      for (int r = 0; r < kRt; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+        for (int c = 0; c < kRt; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
     }
   }
 
   unsigned cnt = 0;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < kRt; ++r) {
     const int i = i0 + ty + 16 * r;
     if (i >= n) continue;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kRt; ++c) {
       const int j = j0 + tx + 16 * c;
       if (j >= n) continue;
       store_pair(A, n, i, j, mirror, pos, g, es, acc[r][c], cnt);
@@ -148,9 +171,18 @@ int launch_table(pycd_ctx* ctx, int precision, const double* pos, int n,
     PYCD_HIP(ctx, hipGetLastError());
   }
   PYCD_HIP(ctx, hipEventRecord(after_sf, s));
-  const int nb = (n + kTile - 1) / kTile;
-  table_f64<<<nb * (nb + 1) / 2, 256, 0, s>>>(Y.ptr, n, Kp, pos, geo, es, out,
-                                              pairs);
+  // tile variant: 1 = 64 x 64, 2 = 128 x 128; 0 picks 128 from 2048 sites
+  int v = ctx->table_variant[PYCD_PREC_F64];
+  if (v == 0) v = n >= 2048 ? 2 : 1;
+  if (v == 2) {
+    const int nb = (n + 127) / 128;
+    table_f64<8><<<nb * (nb + 1) / 2, 256, 0, s>>>(Y.ptr, n, Kp, pos, geo, es,
+                                                   out, pairs);
+  } else {
+    const int nb = (n + 63) / 64;
+    table_f64<4><<<nb * (nb + 1) / 2, 256, 0, s>>>(Y.ptr, n, Kp, pos, geo, es,
+                                                   out, pairs);
+  }
   PYCD_HIP(ctx, hipGetLastError());
   // Y is released when this returns; the stream must have consumed it.
   PYCD_HIP(ctx, hipStreamSynchronize(s));

@@ -160,13 +160,13 @@ table_mfma_f32(const float* __restrict__ Y, int n, int Kp,
   count_pairs(cnt, pairs);
 }
 
-// K is staged in slices of 64 (kBSlice): two 32-deep MFMA steps per slice
-// between barriers, and a slice's global loads (in registers while the
-// previous slice is multiplied) have twice the matrix work to arrive in
-// than 32-deep slices gave them.  Padded rows of 72 bf16 (36 dwords) put
-// the 8 rows one 16-byte LDS read serves on distinct banks.
-constexpr int kBSlice = 64;
-
+// K is staged in slices of kBSlice = 32 or 64: a 64-deep slice makes two
+// 32-deep MFMA steps per pair of barriers, and its global loads (in
+// registers while the previous slice is multiplied) have twice the matrix
+// work to arrive in.  Padded rows of kBSlice + 8 bf16 put the 8 rows one
+// 16-byte LDS read serves on distinct banks.  The sums over k run in the
+// same order for both, so they give the same table bit for bit.
+template <int kBSlice>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(2, 2)))
 table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
@@ -190,17 +190,18 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  // loader: 128 rows x 64 bf16 = 256 threads x 32 bf16 (four 16-byte
-  // chunks) per array
-  const int lrow = t >> 1, lk = (t & 1) * 32;
+  // loader: 128 rows x kBSlice bf16 = 256 threads x kBSlice / 2 bf16
+  // (16-byte chunks) per array
+  constexpr int kQ = kBSlice / 16;  // chunks per thread and array
+  const int lrow = t >> 1, lk = (t & 1) * (kBSlice / 2);
   const bool a_ok = i0 + lrow < n, b_ok = j0 + lrow < n;
   const int64_t aoff = int64_t(i0 + lrow) * Kp + lk;
   const int64_t boff = int64_t(j0 + lrow) * Kp + lk;
   const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
-  uint4 ah[4], al[4], bh[4], bl[4];
+  uint4 ah[kQ], al[kQ], bh[kQ], bl[kQ];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kQ; ++q) {
       ah[q] = a_ok ? load16_global(Yhi + aoff + k0 + 8 * q) : zero;
       al[q] = a_ok ? load16_global(Ylo + aoff + k0 + 8 * q) : zero;
       bh[q] = b_ok ? load16_global(Yhi + boff + k0 + 8 * q) : zero;
@@ -209,7 +210,7 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
   };
   auto sstore = [&]() {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kQ; ++q) {
       *reinterpret_cast<uint4*>(Ah + lrow * kRow + lk + 8 * q) = ah[q];
       *reinterpret_cast<uint4*>(Al + lrow * kRow + lk + 8 * q) = al[q];
       *reinterpret_cast<uint4*>(Bh + lrow * kRow + lk + 8 * q) = bh[q];
@@ -278,7 +279,9 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
                       hipEvent_t after_sf) {
   hipStream_t s = ctx->stream;
   // K padded to whole slices of the mode's kernel (zero columns add nothing)
-  const int slice = precision == PYCD_PREC_F32 ? kMSlice : kBSlice;
+  const int bvar = ctx->table_variant[PYCD_PREC_BF16X3];
+  const int bslice = bvar == 1 ? 32 : 64;
+  const int slice = precision == PYCD_PREC_F32 ? kMSlice : bslice;
   const int Kp = std::max(slice, ((2 * K + slice - 1) / slice) * slice);
   const size_t elems = size_t(n) * Kp;
   DevBuf<float> y32;
@@ -306,9 +309,12 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
   if (precision == PYCD_PREC_F32)
     table_mfma_f32<<<tiles, 256, 0, s>>>(y32.ptr, n, Kp, pos, geo, es, out,
                                          pairs);
+  else if (bslice == 32)
+    table_mfma_bf16x3<32><<<tiles, 256, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, pos,
+                                                geo, es, out, pairs);
   else
-    table_mfma_bf16x3<<<tiles, 256, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, pos,
-                                            geo, es, out, pairs);
+    table_mfma_bf16x3<64><<<tiles, 256, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, pos,
+                                                geo, es, out, pairs);
   PYCD_HIP(ctx, hipGetLastError());
   PYCD_HIP(ctx, hipStreamSynchronize(s));
   return PYCD_OK;

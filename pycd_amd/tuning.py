@@ -1,6 +1,11 @@
-"""Schedules for the batched kernel, chosen by measurement.
+"""Schedules and tile shapes, chosen by measurement.
 
-Two knobs decide how trajectories map onto the chip:
+`autotune_table` times the table kernels' tile variants for one
+precision on the caller's own lattice (the table build is timed on the
+device) and keeps the fastest in the context; every variant gives the same
+table bit for bit.
+
+Two knobs decide how trajectories map onto the chip in the batched kernel:
   * the launch order of the work queue -- longest expected work first
     (LPT).  The library deals this order out to one bin per SIMD in
     serpentine order, so each SIMD gets a mix of long and short
@@ -60,3 +65,28 @@ def autotune_waves(batch, trajs, occs, words, order, candidates=(2, 4, 6, 8,
             log(f"  autotune waves/CU={w}: {results[w]:.4e} hops/s")
     best = max(results, key=results.get)
     return best, results
+
+
+def autotune_table(pos, cell, trans, pbc, alpha, r_cut, k_cut, eps_r,
+                   precision="f64", repeats=2, log=None):
+    """Build the table once per tile variant of `precision` (after one
+    warm-up build), keep the variant whose device time (structure factors,
+    contraction and epilogue) is shortest, and set it for later
+    `pair_table` calls.  Returns (variant, {variant: ms})."""
+    from . import _native
+    times = {}
+    out = None
+    for v in _native.TABLE_VARIANTS[precision]:
+        _native.set_table_variant(precision, v)
+        best = float("inf")
+        for r in range(repeats + 1):
+            out, rep = _native.pair_table(pos, cell, trans, pbc, alpha, r_cut,
+                                          k_cut, eps_r, precision, out=out)
+            if r > 0:  # the first build of a variant warms it up
+                best = min(best, rep["ms_total"])
+        times[v] = best
+        if log:
+            log(f"  autotune table {precision} variant {v}: {best:.2f} ms")
+    chosen = min(times, key=times.get)
+    _native.set_table_variant(precision, chosen)
+    return chosen, times

@@ -139,3 +139,32 @@ def test_precision_configuration_10x10x10():
         rel = abs(clocks[mode] / clocks["f64"] - 1)
         print(f"{mode}: {n} hops, clock {rel:.2e} from the FP64 table's")
         assert rel < 0.05
+
+
+def test_table_variants_agree_and_autotune_picks_one():
+    """Each precision's tile variants (FP64: 64 x 64 and 128 x 128 tiles;
+    BF16x3: K slices of 32 and 64) give the same table bit for bit, and the
+    autotuner keeps the fastest of them in the context."""
+    from pycd_amd import tuning
+    w = workloads.corundum(6)
+    pos = w.neighbors.bulk_sites.cell_coordinates
+    args = (pos, w.neighbors.simulation_cell_matrix,
+            w.system.translational_matrix, w.system.pbc, w.system.alpha,
+            w.system.r_cut, w.system.k_cut, 1.0)
+    try:
+        for mode in ("f64", "bf16x3"):
+            tabs = []
+            for v in _native.TABLE_VARIANTS[mode]:
+                _native.set_table_variant(mode, v)
+                assert _native.table_variant(mode) == v
+                tabs.append(_native.pair_table(*args, mode)[0].copy())
+            for t in tabs[1:]:
+                assert np.array_equal(t, tabs[0]), mode
+            best, times = tuning.autotune_table(*args, precision=mode)
+            print(mode, {k: round(v, 3) for k, v in times.items()}, best)
+            assert best in _native.TABLE_VARIANTS[mode]
+            assert _native.table_variant(mode) == best
+            assert times[best] == min(times.values())
+    finally:
+        for mode in ("f64", "f32", "bf16x3"):
+            _native.set_table_variant(mode, 0)
