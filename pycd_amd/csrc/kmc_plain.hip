@@ -515,40 +515,62 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
       off = uint32_t(x) * a.row_bytes + uint32_t(y) * 8u;
     return *(gmem<double>)(a.tab + off);
   };
-  // psi of a row: the FMA chain over carriers 0 .. n_c - 1
-  auto row_sum = [&](int row) -> double {
-    double acc = 0.0;
-    if (a.uq) {
-      const double q = a.q0;
-      for (int p = 0; p < ncp; ++p) {
-        const double2 v = G[size_t(p) * R + row];
-        acc = fma(q, v.x, acc);
-        if (2 * p + 1 < nc) acc = fma(q, v.y, acc);
+  // psi of this lane's rows (its processes' target rows and its site row):
+  // for each row the FMA chain over carriers 0 .. n_c - 1, the workgroup
+  // kernel's chain.  The cache entries of all the rows are read kC carrier
+  // pairs at a time before they are summed, so a group waits for the LDS
+  // once (one read per pair waited once per pair).  Rows of lanes without
+  // that process or carrier read row 0 and are not kept.
+  constexpr int kRows = kNpl + 1;
+  constexpr int kC = kRows <= 2 ? 4 : 2;
+  auto row_sums = [&](const int (&row)[kRows], double (&out)[kRows]) {
+    double acc[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
+    for (int p0 = 0; p0 < ncp; p0 += kC) {
+      double2 v[kC][kRows];
+#pragma unroll
+      for (int u = 0; u < kC; ++u) {
+        const int p = p0 + u < ncp ? p0 + u : ncp - 1;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) v[u][r] = G[size_t(p) * R + row[r]];
       }
-    } else {
-      for (int p = 0; p < ncp; ++p) {
-        const double2 v = G[size_t(p) * R + row];
-        acc = fma(cqs[2 * p], v.x, acc);
-        if (2 * p + 1 < nc) acc = fma(cqs[2 * p + 1], v.y, acc);
+#pragma unroll
+      for (int u = 0; u < kC; ++u) {
+        const int p = p0 + u;
+        if (p < ncp) {
+          const double q0 = a.uq ? a.q0 : cqs[2 * p];
+          const double q1 = a.uq ? a.q0 : cqs[2 * p + 1];
+          const bool two = 2 * p + 1 < nc;
+#pragma unroll
+          for (int r = 0; r < kRows; ++r) {
+            acc[r] = fma(q0, v[u][r].x, acc[r]);
+            if (two) acc[r] = fma(q1, v[u][r].y, acc[r]);
+          }
+        }
       }
     }
-    return acc;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) out[r] = acc[r];
   };
 
   // ---- processes of this lane: carrier, slot, target, rate inputs
   int pc[kNpl], pj[kNpl], px[kNpl];
   uint32_t pxk[kNpl];
   bool pv[kNpl];
-  double pqs[kNpl], pdphi[kNpl], ptq[kNpl], pdE[kNpl], plam[kNpl], pi4l[kNpl],
-      pvs[kNpl], pvx[kNpl], pvy[kNpl], pvz[kNpl], pst[kNpl];
+  double pqs[kNpl], pdphi[kNpl], pts[kNpl], pEn[kNpl], pEs[kNpl], plam[kNpl],
+      pi4l[kNpl], pvs[kNpl], pvx[kNpl], pvy[kNpl], pvz[kNpl], pst[kNpl];
   // the site-dependent part of process i's rate: slot j of site s, whose
-  // target is n
+  // target is n.  Kept as loaded (the self term and both site energies,
+  // combined in the rate), so that nothing waits for these loads before
+  // the next hop's rates need them.
   auto load_inputs = [&](int i, int s, int j, int n) {
     const SlotRec& r = a.rec[int64_t(s) * a.mz + j];
     const double2 ps = a.pe[s], pn = a.pe[n];
     pdphi[i] = r.dphi;
-    ptq[i] = pqs[i] * r.tself;
-    pdE[i] = pn.y - ps.y;
+    pts[i] = r.tself;
+    pEn[i] = pn.y;
+    pEs[i] = ps.y;
     plam[i] = r.lam;
     pi4l[i] = r.i4l;
     pvs[i] = r.vs;
@@ -595,7 +617,7 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
       pxk[i] = uint32_t(t.y);
       load_inputs(i, s, pj[i], t.x);
     } else {
-      pdphi[i] = ptq[i] = pdE[i] = pvs[i] = 0.0;
+      pdphi[i] = pts[i] = pEn[i] = pEs[i] = pvs[i] = 0.0;
       plam[i] = 1.0;
       pi4l[i] = 0.25;
       pvx[i] = pvy[i] = pvz[i] = 0.0;
@@ -634,10 +656,20 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
   int sx = occ[lane < nc ? lane : 0];
   uint32_t sxk = opk[lane < nc ? lane : 0];
   __syncthreads();
+  // this lane's rows and where their sums go
+  auto lane_sums = [&]() {
+    int row[kRows];
+    double out[kRows];
 #pragma unroll
-  for (int i = 0; i < kNpl; ++i)
-    if (pv[i]) pst[i] = row_sum(lane * kNpl + i);
-  if (lane < nc) psis[lane] = row_sum(nproc + lane);
+    for (int i = 0; i < kNpl; ++i) row[i] = pv[i] ? lane * kNpl + i : 0;
+    row[kNpl] = lane < nc ? nproc + lane : 0;
+    row_sums(row, out);
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i)
+      if (pv[i]) pst[i] = out[i];
+    if (lane < nc) psis[lane] = out[kNpl];
+  };
+  lane_sums();
   // the mover-row gathers this lane takes: row brl (0: the mover's site,
   // 1 + j: its target j) at carrier bc; nb of the kB rounds hold any
   int brl[kB], bc[kB];
@@ -702,7 +734,8 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
       gg[i] = 0.0;
       if (pv[i]) {
         const double term01 = pdphi[i] + (pst[i] - psis[pc[i]]);
-        const double g0 = 2 * pqs[i] * (term01 + ptq[i]) + pdE[i];
+        const double g0 =
+            2 * pqs[i] * (term01 + pqs[i] * pts[i]) + (pEn[i] - pEs[i]);
         kk[i] = marcus_rate(g0, plam[i], pi4l[i], pvs[i], a.nkt, a.nu, ext);
         gg[i] = g0;
         ksum += kk[i];
@@ -785,15 +818,6 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
         gb[k] = entry(x, xk, y, yk);
       }
     }
-#pragma unroll
-    for (int i = 0; i < kNpl; ++i) {
-      if (pv[i] && pc[i] == m) {
-        const int nn = tt[tsel + pj[i]];
-        px[i] = nn;
-        pxk[i] = ttpk[tsel + pj[i]];
-        load_inputs(i, n, pj[i], nn);
-      }
-    }
     const int2 la = a.nbr2[int64_t(tt[tsel + lj]) * a.mz + lk];
 
     // ---- clock, energy and path bookkeeping (uniform), while the loads
@@ -835,6 +859,17 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
       sx = n;
       sxk = npk;
     }
+    // the rate inputs of the mover's new processes (after the uniform work
+    // above, so that it overlaps the loads)
+#pragma unroll
+    for (int i = 0; i < kNpl; ++i) {
+      if (pv[i] && pc[i] == m) {
+        const int nn = tt[tsel + pj[i]];
+        px[i] = nn;
+        pxk[i] = ttpk[tsel + pj[i]];
+        load_inputs(i, n, pj[i], nn);
+      }
+    }
     // ---- the cache: column m of the other rows first, then the mover's
     // rows (which overwrite what the first writes put in them)
 #pragma unroll
@@ -853,10 +888,7 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
       ttpk[(m * z + lj) * z + lk] = uint32_t(la.y);
     }
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < kNpl; ++i)
-      if (pv[i]) pst[i] = row_sum(lane * kNpl + i);
-    if (lane < nc) psis[lane] = row_sum(nproc + lane);
+    lane_sums();
     __syncthreads();
     // ---- path-step outputs, for a hop that ends one or more path steps
     if (row_hi > row_lo) {
