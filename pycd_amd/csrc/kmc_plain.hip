@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <cmath>
 
 namespace pycd {
@@ -486,7 +487,9 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
   extern __shared__ double smem[];
   const int nc = a.nc, z = a.z, nproc = nc * z;
   const int R = nproc + nc;
-  const int ncp = (nc + 1) >> 1;  // carrier pairs of the cache
+  // carrier pairs of the cache, rounded up to whole groups of 4 (the
+  // padding's entries and charges are zero)
+  const int ncp = (((nc + 1) >> 1) + 3) & ~3;
   const int lane = threadIdx.x;
   __shared__ double ext[64];  // 2^(j/64)
   ext[lane] = __builtin_bit_cast(double, kExp2J64[lane]);
@@ -494,8 +497,8 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
   // 16-byte read serves two terms of a row's sum
   double2* G = reinterpret_cast<double2*>(smem);          // ncp x R
   double* psis = reinterpret_cast<double*>(G + size_t(ncp) * R);  // nc
-  double* cqs = psis + nc;                                // nc
-  int32_t* occ = reinterpret_cast<int32_t*>(cqs + nc);    // nc
+  double* cqs = psis + nc;                                // 2 ncp
+  int32_t* occ = reinterpret_cast<int32_t*>(cqs + 2 * ncp);  // nc
   uint32_t* opk = reinterpret_cast<uint32_t*>(occ + nc);  // nc
   int32_t* tt = reinterpret_cast<int32_t*>(opk + nc);     // nproc z
   uint32_t* ttpk = reinterpret_cast<uint32_t*>(tt + nproc * z);
@@ -517,39 +520,39 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
   };
   // psi of this lane's rows (its processes' target rows and its site row):
   // for each row the FMA chain over carriers 0 .. n_c - 1, the workgroup
-  // kernel's chain.  The cache entries of all the rows are read kC carrier
-  // pairs at a time before they are summed, so a group waits for the LDS
-  // once (one read per pair waited once per pair).  Rows of lanes without
-  // that process or carrier read row 0 and are not kept.
+  // kernel's chain (the zero padding adds fma(q, 0, acc) = acc).  The cache
+  // entries of all the rows are read kC carrier pairs at a time before
+  // they are summed, so a group waits for the LDS once.  Rows of lanes
+  // without that process or carrier read row 0 and are not kept.
   constexpr int kRows = kNpl + 1;
-  constexpr int kC = kRows <= 2 ? 4 : 2;
+  constexpr int kC = kRows <= 2 ? 4 : 2;  // divides the padded pair count
   auto row_sums = [&](const int (&row)[kRows], double (&out)[kRows]) {
     double acc[kRows];
 #pragma unroll
     for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
-    for (int p0 = 0; p0 < ncp; p0 += kC) {
-      double2 v[kC][kRows];
+    auto sum = [&](auto uniform_q) {
+      constexpr bool kUq = decltype(uniform_q)::value;
+      for (int p0 = 0; p0 < ncp; p0 += kC) {
+        double2 v[kC][kRows];
 #pragma unroll
-      for (int u = 0; u < kC; ++u) {
-        const int p = p0 + u < ncp ? p0 + u : ncp - 1;
+        for (int u = 0; u < kC; ++u)
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) v[u][r] = G[size_t(p) * R + row[r]];
-      }
+          for (int r = 0; r < kRows; ++r)
+            v[u][r] = G[size_t(p0 + u) * R + row[r]];
 #pragma unroll
-      for (int u = 0; u < kC; ++u) {
-        const int p = p0 + u;
-        if (p < ncp) {
-          const double q0 = a.uq ? a.q0 : cqs[2 * p];
-          const double q1 = a.uq ? a.q0 : cqs[2 * p + 1];
-          const bool two = 2 * p + 1 < nc;
+        for (int u = 0; u < kC; ++u) {
+          const double q0 = kUq ? a.q0 : cqs[2 * (p0 + u)];
+          const double q1 = kUq ? a.q0 : cqs[2 * (p0 + u) + 1];
 #pragma unroll
           for (int r = 0; r < kRows; ++r) {
             acc[r] = fma(q0, v[u][r].x, acc[r]);
-            if (two) acc[r] = fma(q1, v[u][r].y, acc[r]);
+            acc[r] = fma(q1, v[u][r].y, acc[r]);
           }
         }
       }
-    }
+    };
+    if (a.uq) sum(std::true_type{});
+    else sum(std::false_type{});
 #pragma unroll
     for (int r = 0; r < kRows; ++r) out[r] = acc[r];
   };
@@ -585,6 +588,7 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     opk[c] = kCompact ? dv->pk[s] : 0u;
     cqs[c] = dv->cq[c];
   }
+  for (int c = nc + lane; c < 2 * ncp; c += 64) cqs[c] = 0.0;
   // carrier `lane`'s displacement since the last path row, path base and
   // drift (n_c <= 64)
   double dsp[3] = {0.0, 0.0, 0.0}, bse[3] = {0.0, 0.0, 0.0},
@@ -632,8 +636,7 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
     tt[u] = t.x;
     ttpk[u] = uint32_t(t.y);
   }
-  // the cache, every (carrier, row); odd n_c leaves the last pair's second
-  // half zero
+  // the cache, every (carrier, row); the pairs past n_c stay zero
   for (int u = lane; u < ncp * R; u += 64)
     G[u] = make_double2(0.0, 0.0);
   __syncthreads();
@@ -800,24 +803,30 @@ kmc_wave_kernel(WaveArgs a, const KmcDev* __restrict__ dv,
 #pragma unroll
     for (int i = 0; i < kNpl; ++i) ga[i] = entry(px[i], pxk[i], n, npk);
     const double gs_site = entry(sx, sxk, n, npk);
-    double gb[kB];
+    // every round's LDS inputs read first, then every round's gather
+    // issued, unused rounds included (they read a valid entry that is not
+    // kept): no load then sits behind another round's LDS wait, and every
+    // load lands in its own register, so nothing waits for the gathers
+    // before the cache is written
+    int gx[kB], gy[kB];
+    uint32_t gxk[kB], gyk[kB];
 #pragma unroll
     for (int k = 0; k < kB; ++k) {
-      gb[k] = 0.0;
-      if (k < nb) {
-        const int rl = brl[k] > 0 ? brl[k] : 0;
-        int x = n;
-        uint32_t xk = npk;
-        if (rl > 0) {
-          x = tt[tsel + rl - 1];
-          xk = ttpk[tsel + rl - 1];
-        }
-        const int cc = bc[k];
-        const int y = cc == m ? n : occ[cc];
-        const uint32_t yk = cc == m ? npk : opk[cc];
-        gb[k] = entry(x, xk, y, yk);
-      }
+      const int rl = brl[k] > 0 ? brl[k] : 0;
+      const int ti = tsel + (rl > 0 ? rl - 1 : 0);
+      const int tx = tt[ti];
+      const uint32_t txk = ttpk[ti];
+      const int cc = bc[k];
+      const int oy = occ[cc];
+      const uint32_t oyk = opk[cc];
+      gx[k] = rl > 0 ? tx : n;
+      gxk[k] = rl > 0 ? txk : npk;
+      gy[k] = cc == m ? n : oy;
+      gyk[k] = cc == m ? npk : oyk;
     }
+    double gb[kB];
+#pragma unroll
+    for (int k = 0; k < kB; ++k) gb[k] = entry(gx[k], gxk[k], gy[k], gyk[k]);
     const int2 la = a.nbr2[int64_t(tt[tsel + lj]) * a.mz + lk];
 
     // ---- clock, energy and path bookkeeping (uniform), while the loads
@@ -1268,15 +1277,16 @@ hipError_t pack_wave_inputs(pycd_kmc* s) {
   return hipGetLastError();
 }
 
-// LDS of the one-wavefront kernel: the row cache (n_c x (n_c z + n_c)
-// doubles, carriers in pairs), site-row psi and charges (2 n_c doubles),
-// carrier sites and their codes, and each process's look-ahead (2 n_c z z
-// words).  0 when the instance does not apply.
+// LDS of the one-wavefront kernel: the row cache (carrier pairs, rounded
+// up to groups of 4, x (n_c z + n_c) rows of 16 bytes), site-row psi
+// (n_c doubles) and charges (one per padded carrier), carrier sites and
+// their codes, and each process's look-ahead (2 n_c z z words).  0 when
+// the instance does not apply.
 size_t wave_lds_bytes(int nc, int z) {
   const int nproc = nc * z;
   if (nc > 64 || nproc > 256 || z > 8) return 0;
-  const size_t ncp = (size_t(nc) + 1) / 2;
-  return 16 * ncp * (nproc + nc) + sizeof(double) * 2 * size_t(nc) +
+  const size_t ncp = (((size_t(nc) + 1) / 2) + 3) & ~size_t(3);
+  return 16 * ncp * (nproc + nc) + sizeof(double) * (size_t(nc) + 2 * ncp) +
          sizeof(int32_t) * (2 * size_t(nc) + 2 * size_t(nproc) * z);
 }
 
