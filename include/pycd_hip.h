@@ -355,6 +355,15 @@ int pycd_batch_setup(pycd_batch* b, const pycd_kmc_traj* trajs,
  * reference runs trajectories one after another, PyCD/core.py:2668). */
 int pycd_batch_set_handover(pycd_batch* b, int32_t on);
 
+/* Whether launches read the families' hop differences (default on; kept
+ * for each family whose n_unit z x N FP64 rows stay below 4 GiB): the
+ * change of every slot potential when a carrier hops is then one table
+ * gather instead of two, the same number either way.  Off, or for a family
+ * without them, the two entries are gathered.  No reference counterpart
+ * (the reference sums the potential from the full table each hop,
+ * PyCD/core.py:1989-2050). */
+int pycd_batch_set_hop_differences(pycd_batch* b, int32_t on);
+
 /* Runs every trajectory (and the MSD reduction), optionally through a
  * captured hipGraph; *elapsed_ms = device time between events around it. */
 int pycd_batch_run(pycd_batch* b, int32_t use_graph, double* elapsed_ms);

@@ -113,6 +113,7 @@ _SIGNATURES = {
     "pycd_batch_run": (C.c_int, [C.c_void_p, C.c_int32,
                                  C.POINTER(C.c_double)]),
     "pycd_batch_set_handover": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pycd_batch_set_hop_differences": (C.c_int, [C.c_void_p, C.c_int32]),
     "pycd_batch_results": (C.c_int, [C.c_void_p, C.POINTER(TrajResultC),
                                      C.c_void_p]),
     "pycd_batch_rows": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),

@@ -57,10 +57,16 @@ class Family:
         self.alpha = run.system.alpha
 
     def device_bytes(self, table_precision="f64"):
-        """Device footprint of the family's compact form."""
+        """Device footprint of the family's compact form: the cell-0 rows,
+        the hop differences (n_unit z rows of FP64, kept while they stay
+        below 4 GiB), per-sublattice records and per-site slot words."""
         rows = self.n_unit * self.n_sites
         per = 12 if table_precision == "f32" else 8
-        return rows * per + self.n_unit * (8 + 72 * self.z) + 4 * self.n_sites
+        diff = 8 * self.n_unit * self.z * self.n_sites
+        if diff >= 1 << 32:
+            diff = 0
+        return (rows * per + diff + self.n_unit * (8 + 72 * self.z)
+                + 4 * self.n_sites)
 
     def compact(self):
         """The same family with only the table rows of cell 0's sites."""
@@ -158,6 +164,12 @@ class KmcBatch:
         for launches with SIMD bins); results are the same either way."""
         _check(self.ctx, lib().pycd_batch_set_handover(self.h, 1 if on else 0),
                "pycd_batch_set_handover")
+
+    def set_hop_differences(self, on):
+        """Read the families' hop differences (default on): one gather per
+        slot row and hop instead of two; results are the same either way."""
+        _check(self.ctx, lib().pycd_batch_set_hop_differences(
+            self.h, 1 if on else 0), "pycd_batch_set_hop_differences")
 
     def run(self, use_graph=False):
         ms = C.c_double(0.0)

@@ -67,6 +67,10 @@ struct FamDev {
   const uint32_t* perm;   // N: slot j of the site is slot
                           //    (perm >> 3 j) & 7 of its sublattice
   const double* phiL;     // n_unit
+  // n_unit z x N or NULL: row s z + j holds A[n, x] - A[s, x] at the
+  // column of x in s's row, n the target of slot j of sublattice s (the two
+  // entries as the hop loop reads them, FP32 ones in the FP32 table mode)
+  const double* D;
   int64_t N;
   int32_t nx, ny, nz, nu;
   int32_t z, pad;
@@ -166,6 +170,19 @@ using gptr = const __attribute__((address_space(1))) T*;
 template <typename T>
 __device__ inline gptr<T> as_global(const T* p) {
   return (gptr<T>)p;
+}
+// (the host pass of this file sees plain pointers: its compiler has no
+// constant address space to bind the records' copy constructors to)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+#else
+template <typename T>
+using cptr = const T*;
+#endif
+template <typename T>
+__device__ inline cptr<T> as_const(const T* p) {
+  return (cptr<T>)p;
 }
 template <typename T>
 using gmut = __attribute__((address_space(1))) T*;
@@ -556,11 +573,20 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     }
     const bool resume = uni(o0.started) != 0;
     const uint64_t w_start = wall_clock64();
-    const TrajDev td = a.trajs[tr];
-    const FamDev f = a.fams[uni(td.fam)];
+    // The launch never writes the trajectory and family records: read
+    // through the constant address space they are scalar loads, and their
+    // fields (table base, row stride, sizes) live in scalar registers rather
+    // than in vector registers read back with readfirstlane at every use.
+    const TrajDev td = as_const(a.trajs)[tr];
+    const FamDev f = as_const(a.fams)[uni(td.fam)];
     const Geo g{uni(f.nx), uni(f.ny), uni(f.nz), uni(f.nu)};
     const GeoPk gp = geo_pk(g);
     const Tab<TT> tab = tab_of<TT>(f);
+    // the family's hop differences (absent: two table entries per slot row)
+    Tab<double> dtab;
+    dtab.base = (gptr<char>)as_global(f.D);
+    dtab.row_bytes = uint32_t(f.N) * uint32_t(sizeof(double));
+    const bool use_d = f.D != nullptr;
     const int nc = uni(td.nc), z = uni(f.z), zp = z + 1;
     const int R = nc * zp, nproc = nc * z;
     const int per = (nproc + 63) / 64;
@@ -692,8 +718,9 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     // outputs and resumable state: at the trajectory's end, or at a hop
     // boundary when it is handed over
     auto save = [&](bool done_now) {
-      for (int k = lane; k < w; k += 64)
-        a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = sdrift[k];
+      // every carrier slot of the output, zero past the trajectory's own
+      for (int k = lane; k < 3 * kMaxCarriers; k += 64)
+        a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = k < w ? sdrift[k] : 0.0;
       for (int k = lane; k < 624; k += 64) mts[k] = mt[k];
       for (int c = lane; c < nc; c += 64) socc[c] = occ[c];
       for (int r = lane; r < R; r += 64) spsi[r] = psi[r];
@@ -751,7 +778,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       --left;
       ++slice_done;
       if (a.progress && (uint32_t(hops) & 63u) == 0) {
-        pace(a.progress, pslot, left + 1, lane);
+        pace(a.progress, pslot, left + 1, lane_h);
         if (!stolen && left >= kHandMinHops) {
           if (lane_h == 0) box = claim_mailbox(hand_words(a.progress));
           box = uni(__shfl(box, 0, 64));
@@ -851,45 +878,42 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         cum = wave_excl_scan(xs);
         thr = u1;
       }
-      int found = -1;
-      uint32_t d_found = 0;
-      double g_found = 0.0, k_found = 0.0;
+      // The selected process is the first (lane, i), lane-major, whose
+      // running sum cum + xq[0] + ... + xq[i] exceeds thr.  The terms are
+      // non-negative, so each lane's running sums do not decrease: the
+      // count of them that stay <= thr is the index of the lane's first
+      // crossing (a lane crosses iff the count is below per, and its first
+      // process is real: processes past nproc add zero at the lane's end).
+      int nb = 0;
 #pragma unroll
       for (int i = 0; i < kPer; ++i) {
         if (i < per_h) {
           cum += xq[i];
-          const bool hit = found < 0 && (pd[i] >> 31) && cum > thr;
-          found = hit ? lane_h * per_h + i : found;
-          d_found = hit ? pd[i] : d_found;
-          g_found = hit ? gv[i] : g_found;
-          k_found = hit ? kv[i] : k_found;
+          nb += cum > thr ? 0 : 1;
         }
       }
-      const unsigned long long mask = __ballot(found >= 0);
-      int src;
+      const unsigned long long mask =
+          __ballot((pd[0] >> 31) != 0u && nb < per_h);
+      int src, isel;
       if (mask) {
-        src = __ffsll(static_cast<long long>(mask)) - 1;
+        src = uni(__ffsll(static_cast<long long>(mask)) - 1);
+        isel = __builtin_amdgcn_readlane(nb, src);
       } else {
-        // rounding left every cum <= u1: take the last process
+        // rounding left every running sum <= thr: take the last process
         src = (nproc - 1) / per_h;
-        if (lane_h == src) {
-          found = nproc - 1;
-          const int i = nproc - 1 - lane_h * per_h;
-#pragma unroll
-          for (int ii = 0; ii < kPer; ++ii)
-            if (ii == i) {
-              d_found = pd[ii];
-              g_found = gv[ii];
-              k_found = kv[ii];
-            }
-        }
+        isel = nproc - 1 - src * per_h;
         ++no_select;
       }
-      src = uni(src);
-      const int sel = __builtin_amdgcn_readlane(found, src);
-      const uint32_t dsel = uint32_t(__builtin_amdgcn_readlane(int(d_found), src));
-      const double gsel = read_lane(g_found, src);
-      const double ksel = read_lane(k_found, src);
+      const int sel = src * per_h + isel;
+      uint32_t dsel = 0;
+      double gsel = 0.0, ksel = 0.0;
+#pragma unroll
+      for (int i = 0; i < kPer; ++i)
+        if (i == isel) {
+          dsel = uint32_t(__builtin_amdgcn_readlane(int(pd[i]), src));
+          gsel = read_lane(gv[i], src);
+          ksel = read_lane(kv[i], src);
+        }
       // ---- clock
       int32_t end;
       if (lu2 > -INFINITY) {  // u2 > 0
@@ -935,10 +959,15 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
                           : 0u;
       const uint32_t rn = tab.row(sub_new), ro = tab.row(s_old >> 24);
       const RowKey key_new = row_key(s_new, gp), key_old = row_key(s_old, gp);
+      const uint32_t rd = dtab.row(rec_old);
       constexpr int kRPer = kMaxCarriers * kZp / 64;
-      // both entries of every slot are loaded before any is used, so that
-      // the slots' gathers are in flight together
-      TT an[kRPer], ao[kRPer];
+      // Every slot's change q (A[n, x] - A[s, x]) is loaded before any is
+      // used, so that the slots' gathers are in flight together: one entry
+      // of the family's hop differences (row rec_old, x's column in s_old's
+      // row) where it has them, else both table entries (an - double(ao),
+      // ao = 0 with differences)
+      double an[kRPer];
+      TT ao[kRPer];
       // the slot sites of this lane's rows, read together (one LDS wait,
       // not one per row)
       uint32_t xs[kRPer];
@@ -950,11 +979,15 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
         const int r = lane_h + 64 * i;
-        an[i] = TT(0);
-        ao[i] = TT(0);
+        // set and read only for r < R (no zero fill of unused rows)
         if (r < R_h) {
-          an[i] = tab.at(rn, rel_col(key_new, xs[i], gp));
-          ao[i] = tab.at(ro, rel_col(key_old, xs[i], gp));
+          if (use_d) {
+            an[i] = dtab.at(rd, rel_col(key_old, xs[i], gp));
+            ao[i] = TT(0);
+          } else {
+            an[i] = double(tab.at(rn, rel_col(key_new, xs[i], gp)));
+            ao[i] = tab.at(ro, rel_col(key_old, xs[i], gp));
+          }
         }
       }
       // fresh sums of the new slots (not the one that returns to s_old):
@@ -987,7 +1020,7 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
 #pragma unroll
       for (int i = 0; i < kRPer; ++i) {
         const int r = lane_h + 64 * i;
-        if (r < R_h) psi[r] += q * (double(an[i]) - double(ao[i]));
+        if (r < R_h) psi[r] += q * (an[i] - double(ao[i]));
       }
       if (mv) {
 #pragma clang fp contract(off)
@@ -1325,7 +1358,7 @@ compact_check(const double* __restrict__ A, int64_t N, Geo g,
 using namespace pycd;
 
 struct BatchFamily {
-  DevBuf<double> C64, phiL;
+  DevBuf<double> C64, phiL, D;
   DevBuf<float> C32;
   DevBuf<SubRec> subs;
   DevBuf<uint32_t> perm;
@@ -1342,6 +1375,7 @@ struct pycd_batch {
   std::vector<BatchFamily*> fams;
   DevBuf<FamDev> d_fams;
   int table_prec = 0;  // 0 FP64, 1 FP32
+  bool use_diff = true;  // launches read the families' hop differences
   int64_t refresh = 1024;   // set from the rate precision at creation
   int rate_prec = 0;   // 0 FP64, 1 FP32
   double nu = 0.0;

@@ -220,6 +220,44 @@ def test_handover_leaves_results_unchanged():
             assert np.array_equal(a, b)
 
 
+def test_hop_differences_leave_results_unchanged():
+    """With its families' hop differences a launch gathers each slot's
+    potential change A[n, x] - A[s, x] as one entry; without them (switch
+    off, or a family too large to hold them) it gathers both entries.  The
+    numbers are the same: every result equal bit for bit, for FP64 and FP32
+    rates and the FP32 table, on corundum and monoclinic pairs of families
+    in one launch (1100 trajectories over per-SIMD bins, so the XCD shares
+    of the queue hold different families)."""
+    for builder, sizes in ((workloads.corundum, [3, 4]),
+                           (workloads.monoclinic, [4, 5])):
+        fams, runs = _families(sizes, builder=builder, t_final=2e-8)
+        rs = np.random.default_rng(31)
+        n = 1100
+        specs = [(int(f), int(nc), float(fv), float(T))
+                 for f, nc, fv, T in zip(rs.integers(0, 2, n),
+                                         rs.integers(1, 17, n),
+                                         rs.choice([0.0, 1e6, 5e6], n),
+                                         rs.uniform(250.0, 400.0, n))]
+        trajs, occs, words = _mixed_batch(fams, runs, specs, 41, 300)
+        order = tuning.lpt_order(trajs, fams)
+        for table, rates in (("f64", "f64"), ("f64", "f32"), ("f32", "f32")):
+            out = []
+            for on in (True, False):
+                bt = B.KmcBatch(fams, table, rates)
+                bt.set_hop_differences(on)
+                bt.setup(trajs, occs, words, order=order, waves_per_cu=16)
+                bt.run()
+                out.append((bt.results(),
+                            [bt.rows(k) for k in (0, 1, 550, n - 1)]))
+                bt.close()
+            (ra, rows_a), (rb, rows_b) = out
+            assert np.all(ra["hops"] > 0)
+            for key in ("t", "hops", "energy", "start", "drift", "no_select"):
+                assert np.array_equal(ra[key], rb[key]), (table, rates, key)
+            for a, b in zip(rows_a, rows_b):
+                assert np.array_equal(a, b)
+
+
 def test_fp64_batch_monoclinic_matches_oracle_per_trajectory():
     """The serving form's monoclinic lattice (C-centred, 4 sites per cell,
     two hop slots) in the batched FP64 mode, against the oracle."""

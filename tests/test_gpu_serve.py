@@ -57,13 +57,13 @@ def test_same_alone_and_in_the_mix():
 
 
 def test_table_budget_forces_eviction():
-    """Compact families of 324 and 768 sites (36.0 kB and 80.4 kB on the
-    device) do not fit a 100 kB budget together: sizes alternate, so
-    families are evicted and rebuilt, and every job still completes.  A job
-    whose family alone exceeds the budget (1500 sites, 153.6 kB) is
-    rejected instead of waiting forever."""
+    """Compact families of 324 and 768 sites (160.4 kB and 375.3 kB on the
+    device with their hop differences) do not fit a 400 kB budget together:
+    sizes alternate, so families are evicted and rebuilt, and every job
+    still completes.  A job whose family alone exceeds the budget (1500
+    sites, 729.6 kB) is rejected instead of waiting forever."""
     srv = KmcServer(corundum_family, n_slots=16, nc_max=8,
-                    rows_capacity=1 << 20, table_budget_bytes=100_000,
+                    rows_capacity=1 << 20, table_budget_bytes=400_000,
                     slice_hops=500, path_rows=20, msd_lags=5)
     jobs = [Job(k, edge=3 if k % 2 == 0 else 4, n_carriers=2, hops=1000,
                 seed=k) for k in range(8)]
@@ -71,9 +71,9 @@ def test_table_budget_forces_eviction():
     for j in jobs:
         srv.submit(j)
     srv.submit(big)
-    assert srv.table_bytes(3) + srv.table_bytes(4) > 100_000
-    assert max(srv.table_bytes(3), srv.table_bytes(4)) <= 100_000
-    assert srv.table_bytes(5) > 100_000
+    assert srv.table_bytes(3) + srv.table_bytes(4) > 400_000
+    assert max(srv.table_bytes(3), srv.table_bytes(4)) <= 400_000
+    assert srv.table_bytes(5) > 400_000
     srv.drain()
     assert all(j.result is not None and 0 < j.result["hops"] <= 1000
                for j in jobs)
