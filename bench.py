@@ -81,13 +81,14 @@ def parse():
 
 
 def algorithmic_bytes_batched(trajs, fams, hops, starts, table_bytes,
-                              refresh=1024):
+                              refresh=1024, diffs=True):
     """Bytes the batched algorithm touches per launch.
 
     Per hop of a trajectory with n_c carriers and z slots:
-      table gathers   2 (1 + z) n_c + (z - 1) n_c entries (every slot's psi
-                      update, fresh sums of the mover's new slots), plus
-                      (1 + z) n_c^2 / refresh for the periodic recompute
+      slot updates    (1 + z) n_c FP64 hop differences, one per slot row
+                      (with diffs=False: 2 (1 + z) n_c table entries)
+      fresh sums      (z - 1) n_c table entries (the mover's new slots)
+      refresh         (1 + z) n_c^2 / refresh table entries
       hop records     z x 72 B (the new processes) + 8 B (hop vector)
       mover state     2 x 48 B (displacement and drift, read and written)
     plus 3 n_c doubles per path row written.
@@ -96,8 +97,9 @@ def algorithmic_bytes_batched(trajs, fams, hops, starts, table_bytes,
     for k, t in enumerate(trajs):
         z = fams[t["family"]].z
         nc = t["n_carriers"]
-        entries = (2 * (1 + z) + (z - 1)) * nc + (1 + z) * nc * nc / refresh
-        per_hop = entries * table_bytes + z * 72 + 8 + 96
+        upd = (1 + z) * nc * 8.0 if diffs else 2 * (1 + z) * nc * table_bytes
+        entries = (z - 1) * nc + (1 + z) * nc * nc / refresh
+        per_hop = upd + entries * table_bytes + z * 72 + 8 + 96
         total += per_hop * hops[k] + 24.0 * nc * starts[k]
     return total
 

@@ -1116,11 +1116,14 @@ kmc_pool_kernel(BatchArgs a) {
 //   msd[t, tau] = mean_c mean_{t0 < P_t - tau} |r_c(t0 + tau) - r_c(t0)|^2
 // (P_t: the path rows the run wrote).  Two kernels.
 //
-// msd_unit_kernel: one wavefront per (trajectory, carrier) unit.  The
-// carrier's positions pass through LDS (x, y and z arrays) in windows read
-// from global memory once each.  A pass takes kMsdPass lags, lane l the
-// kMsdL consecutive lags tau0 + kMsdL l + i; the lane walks the origins t0
-// of a window with r(t0 + kMsdL l + i), i = 0 .. kMsdL, in a ring of
+// msd_unit_kernel: one workgroup of kMsdWaves wavefronts per (trajectory,
+// carrier) unit.  The carrier's positions pass through LDS (x, y and z
+// arrays) in windows read from global memory once each; the wavefronts
+// take consecutive quarters of a window's origins and their lag sums are
+// added in wavefront order at the end of a pass (so that one window serves
+// four wavefronts and a CU holds 16 of them).  A pass takes kMsdPass lags,
+// lane l the kMsdL consecutive lags tau0 + kMsdL l + i; the lane walks its
+// origins t0 with r(t0 + kMsdL l + i), i = 0 .. kMsdL, in a ring of
 // registers, so a step reads the origin (one address for the whole
 // wavefront) and one new position, and forms kMsdL squared distances.  The
 // odd stride kMsdL = 9 puts the 16 lanes of an LDS access on distinct banks.
@@ -1150,19 +1153,27 @@ constexpr int kMsdL = 9;               // lags per lane and pass
 constexpr int kMsdPass = 64 * kMsdL;   // lags per pass
 constexpr int kMsdWin = 1024;          // path rows held in LDS
 constexpr int kMsdXcds = 8;
+constexpr int kMsdWaves = 4;           // wavefronts per unit
+constexpr int kMsdThreads = 64 * kMsdWaves;
 
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kMsdThreads)
+__attribute__((amdgpu_waves_per_eu(4, 4)))
 msd_unit_kernel(const double* __restrict__ rows,
                 const MsdTraj* __restrict__ tr,
                 const TrajOut* __restrict__ outs,
                 const int32_t* __restrict__ units,
                 double* __restrict__ scratch) {
-  __shared__ double X[kMsdWin], Y[kMsdWin], Z[kMsdWin];
+  // the window; after a pass its space takes the wavefronts' lag sums
+  __shared__ double W[3 * kMsdWin];
+  double* X = W;
+  double* Y = W + kMsdWin;
+  double* Z = W + 2 * kMsdWin;
   const int32_t u = units[blockIdx.x];
-  if (u < 0) return;  // padding of an XCD's list
+  if (u < 0) return;  // padding of an XCD's list (the whole workgroup)
   const MsdTraj d = tr[u >> 6];
   const int c = u & 63;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
   // only the rows written by the run: [0, start) (all P when the path filled)
   const int64_t written = outs[d.traj].start;
   const int64_t P = written < 1 ? 1 : (written < d.P ? written : d.P);
@@ -1187,23 +1198,27 @@ msd_unit_kernel(const double* __restrict__ rows,
       const int64_t wlo = T0 + pass0;
       const int nw = int(min<int64_t>(P, T1 - 1 + pass0 + span) - wlo);
       __syncthreads();
-      for (int r = lane; r < nw; r += 64) {
+      for (int r = tid; r < nw; r += kMsdThreads) {
         const int64_t o = (wlo + r) * w3;
         X[wofs + r] = src[o];
         Y[wofs + r] = src[o + 1];
         Z[wofs + r] = src[o + 2];
       }
       if (!first)
-        for (int r = lane; r < T1 - T0; r += 64) {
+        for (int r = tid; r < T1 - T0; r += kMsdThreads) {
           const int64_t o = (T0 + r) * w3;
           X[r] = src[o];
           Y[r] = src[o + 1];
           Z[r] = src[o + 2];
         }
       __syncthreads();
-      // every lane walks the same origins (its ring slots line up with the
-      // others'); a lag whose row lies past the written rows, or past M,
-      // adds nothing (lags past M are dropped at the end)
+      // wavefront `wave` takes the origins [ta, tb) of the segment
+      const int64_t q = (T1 - T0 + kMsdWaves - 1) / kMsdWaves;
+      const int64_t ta = min<int64_t>(T1, T0 + wave * q);
+      const int64_t tb = min<int64_t>(T1, ta + q);
+      // every lane of a wavefront walks the same origins (its ring slots
+      // line up with the others'); a lag whose row lies past the written
+      // rows, or past M, adds nothing (lags past M are dropped at the end)
       const int rel = int(lo - pass0) + wofs;  // LDS row of r(T0 + lo)
       double wx[kMsdL + 1], wy[kMsdL + 1], wz[kMsdL + 1];
       auto ld = [&](int row, double& x, double& y, double& z) {
@@ -1212,15 +1227,16 @@ msd_unit_kernel(const double* __restrict__ rows,
         y = Y[rr];
         z = Z[rr];
       };
+      const int base = rel + int(ta - T0);  // LDS row of r(ta + lo)
 #pragma unroll
-      for (int i = 0; i < kMsdL; ++i) ld(rel + i, wx[i], wy[i], wz[i]);
+      for (int i = 0; i < kMsdL; ++i) ld(base + i, wx[i], wy[i], wz[i]);
       // the highest first lag among lanes with a lag inside the pass and the
       // written rows: below fast_end every such lane has all its rows
       const int64_t reach = min<int64_t>(span, P - pass0);
       const int64_t lo_top = pass0 + kMsdL * ((reach - 1) / kMsdL);
-      const int64_t fast_end = min<int64_t>(T1, P - (lo_top + kMsdL - 1));
+      const int64_t fast_end = min<int64_t>(tb, P - (lo_top + kMsdL - 1));
       constexpr int kR = kMsdL + 1;  // ring slots; a turn is kR origins
-      int64_t t0 = T0;
+      int64_t t0 = ta;
       // Whole turns: at step j of a turn slot (p % kR) holds
       // r(t0 + lo + p) for p = j .. j + kMsdL - 1; the step loads position
       // j + kMsdL into the slot position j - 1 left
@@ -1233,18 +1249,18 @@ msd_unit_kernel(const double* __restrict__ rows,
           const double ox = X[oi], oy = Y[oi], oz = Z[oi];
 #pragma unroll
           for (int i = 0; i < kMsdL; ++i) {
-            const int q = (j + i) % kR;
-            const double dx = wx[q] - ox, dy = wy[q] - oy, dz = wz[q] - oz;
+            const int qq = (j + i) % kR;
+            const double dx = wx[qq] - ox, dy = wy[qq] - oy, dz = wz[qq] - oz;
             acc[i] = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[i])));
           }
         }
       }
       // the rest: lags checked against the written rows, origins against
-      // the segment's end
-      for (; t0 < T1; t0 += kR) {
+      // the chunk's end
+      for (; t0 < tb; t0 += kR) {
 #pragma unroll
         for (int j = 0; j < kR; ++j) {
-          if (t0 + j >= T1) break;
+          if (t0 + j >= tb) break;
           const int oi = int(t0 - T0) + j;
           ld(rel + oi + kMsdL, wx[(j + kMsdL) % kR], wy[(j + kMsdL) % kR],
              wz[(j + kMsdL) % kR]);
@@ -1252,18 +1268,24 @@ msd_unit_kernel(const double* __restrict__ rows,
           const int64_t nv = P - (t0 + j + lo);  // lags i < nv have rows
 #pragma unroll
           for (int i = 0; i < kMsdL; ++i) {
-            const int q = (j + i) % kR;
-            const double dx = wx[q] - ox, dy = wy[q] - oy, dz = wz[q] - oz;
+            const int qq = (j + i) % kR;
+            const double dx = wx[qq] - ox, dy = wy[qq] - oy, dz = wz[qq] - oz;
             const double a = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[i])));
             acc[i] = i < nv ? a : acc[i];
           }
         }
       }
     }
-    if (lo < d.M) {
+    // the wavefronts' sums of each lag added in wavefront order
+    __syncthreads();
 #pragma unroll
-      for (int i = 0; i < kMsdL; ++i)
-        if (lo + i < d.M) out[lo + i] = acc[i];
+    for (int i = 0; i < kMsdL; ++i) W[wave * kMsdPass + kMsdL * lane + i] = acc[i];
+    __syncthreads();
+    for (int l = tid; l < span; l += kMsdThreads) {
+      double v = W[l];
+#pragma unroll
+      for (int w = 1; w < kMsdWaves; ++w) v += W[w * kMsdPass + l];
+      out[pass0 + l] = v;
     }
   }
 }
