@@ -640,10 +640,17 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     }
     if (!resume)
       for (int k = lane; k < w; k += 64) {
-        sdisp[k] = 0.0;
-        sdrift[k] = 0.0;
         sbase[k] = 0.0;
         rows[k] = 0.0;  // path row 0: the initial positions
+      }
+    // displacement since the last path row and drift of carrier `lane`, in
+    // its lane while the trajectory runs (saved with the state)
+    double dsp[3] = {0.0, 0.0, 0.0}, drf[3] = {0.0, 0.0, 0.0};
+    if (resume && lane < nc)
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) {
+        dsp[c3] = sdisp[3 * lane + c3];
+        drf[c3] = sdrift[3 * lane + c3];
       }
     __syncthreads();
     for (int r = lane; r < R; r += 64) {
@@ -719,8 +726,16 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     // boundary when it is handed over
     auto save = [&](bool done_now) {
       // every carrier slot of the output, zero past the trajectory's own
-      for (int k = lane; k < 3 * kMaxCarriers; k += 64)
-        a.drift[int64_t(tr) * 3 * kMaxCarriers + k] = k < w ? sdrift[k] : 0.0;
+      static_assert(kMaxCarriers == 64, "one carrier per lane");
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) {
+        a.drift[int64_t(tr) * 3 * kMaxCarriers + 3 * lane + c3] =
+            lane < nc ? drf[c3] : 0.0;
+        if (lane < nc) {
+          sdisp[3 * lane + c3] = dsp[c3];
+          sdrift[3 * lane + c3] = drf[c3];
+        }
+      }
       for (int k = lane; k < 624; k += 64) mts[k] = mt[k];
       for (int c = lane; c < nc; c += 64) socc[c] = occ[c];
       for (int r = lane; r < R; r += 64) spsi[r] = psi[r];
@@ -1005,16 +1020,10 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
             fent[k] = tab.at(tab.row(nsel[k] >> 24), rel_col(nsel[k], o, gp));
         }
       }
-      // hop vector of carrier m, and its displacement (lanes 0..2) and
-      // drift (lanes 3..5, with a field) so far, one component per lane
-      const int hc = lane_h < 3 ? lane_h : lane_h - 3;
-      const bool mv = lane_h < 3 || (lane_h < 6 && field_on);
-      const int64_t vo = (lane_h < 3 ? 0 : 6 * kMaxCarriers) + 3 * m + hc;
-      double vec_own = 0.0, v_acc = 0.0;
-      if (mv) {
-        vec_own = ptab[rec_old].vec[hc];
-        v_acc = svec_g[vo];
-      }
+      // hop vector of carrier m (the same address in every lane)
+      double hv[3];
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) hv[c3] = ptab[rec_old].vec[c3];
       // ---- apply (process m's old entries are dead once it is selected)
       if (lane_h < z_h) pinfo[m * z_h + lane_h] = info_new;
 #pragma unroll
@@ -1022,9 +1031,13 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
         const int r = lane_h + 64 * i;
         if (r < R_h) psi[r] += q * (an[i] - double(ao[i]));
       }
-      if (mv) {
+      if (lane_h == m) {
 #pragma clang fp contract(off)
-        svec_g[vo] = v_acc + (lane_h < 3 ? vec_own : vec_own * ksel);
+#pragma unroll
+        for (int c3 = 0; c3 < 3; ++c3) {
+          dsp[c3] += hv[c3];
+          if (field_on) drf[c3] = drf[c3] + hv[c3] * ksel;
+        }
       }
       __syncthreads();
       const double v_new = uni(psi[m * zp + 1 + j]);
@@ -1055,12 +1068,16 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
       bool full = false;
       if (end >= start + 1) {
         if (end >= P) end = P;
-        for (int k = lane_h; k < w; k += 64) {
-          const double v = sbase[k] + sdisp[k];
-          sdisp[k] = 0.0;
-          for (int32_t r = start; r < end; ++r) rows[int64_t(r) * w + k] = v;
-          sbase[k] = v;
-        }
+        if (lane_h < nc_h)
+#pragma unroll
+          for (int c3 = 0; c3 < 3; ++c3) {
+            const int k = 3 * lane_h + c3;
+            const double v = sbase[k] + dsp[c3];
+            dsp[c3] = 0.0;
+            for (int32_t r = start; r < end; ++r)
+              rows[int64_t(r) * w + k] = v;
+            sbase[k] = v;
+          }
         start = end;
         full = end >= P;
       }
