@@ -81,7 +81,7 @@ def parse():
 
 
 def algorithmic_bytes_batched(trajs, fams, hops, starts, table_bytes,
-                              refresh=1024, diffs=True):
+                              refresh=4096, diffs=True):
     """Bytes the batched algorithm touches per launch.
 
     Per hop of a trajectory with n_c carriers and z slots:
@@ -90,8 +90,8 @@ def algorithmic_bytes_batched(trajs, fams, hops, starts, table_bytes,
       fresh sums      (z - 1) n_c table entries (the mover's new slots)
       refresh         (1 + z) n_c^2 / refresh table entries
       hop records     z x 72 B (the new processes) + 8 B (hop vector)
-      mover state     2 x 48 B (displacement and drift, read and written)
-    plus 3 n_c doubles per path row written.
+    plus 3 n_c doubles per path row written (the carriers' displacement and
+    drift stay in their lanes between path rows).
     """
     total = 0.0
     for k, t in enumerate(trajs):
@@ -99,7 +99,7 @@ def algorithmic_bytes_batched(trajs, fams, hops, starts, table_bytes,
         nc = t["n_carriers"]
         upd = (1 + z) * nc * 8.0 if diffs else 2 * (1 + z) * nc * table_bytes
         entries = (z - 1) * nc + (1 + z) * nc * nc / refresh
-        per_hop = upd + entries * table_bytes + z * 72 + 8 + 96
+        per_hop = upd + entries * table_bytes + z * 72 + 8
         total += per_hop * hops[k] + 24.0 * nc * starts[k]
     return total
 
