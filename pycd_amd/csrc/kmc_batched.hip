@@ -1142,8 +1142,10 @@ kmc_pool_kernel(BatchArgs a) {
 // lane l the kMsdL consecutive lags tau0 + kMsdL l + i; the lane walks its
 // origins t0 with r(t0 + kMsdL l + i), i = 0 .. kMsdL, in a ring of
 // registers, so a step reads the origin (one address for the whole
-// wavefront) and one new position, and forms kMsdL squared distances.  The
-// odd stride kMsdL = 9 puts the 16 lanes of an LDS access on distinct banks.
+// wavefront) and one new position, and forms kMsdL squared distances.  Rows
+// are held with one pad double after every 8, so the rows of lanes l and
+// l + 1 (8 apart) sit 9 doubles apart and the 16 lanes of an LDS access
+// fall on distinct banks; kMsdL = 8 leaves one lane idle for 500 lags.
 // For the first pass the window holds rows [T0, T0 + S + span - 1), which
 // contain the segment's S origins; later passes keep the origins in their
 // own S rows ahead of the window.  Sums over origins, unscaled, per (unit,
@@ -1166,9 +1168,12 @@ struct MsdTraj {
   int64_t sbase;    // scratch offset of carrier 0's M lag sums
 };
 
-constexpr int kMsdL = 9;               // lags per lane and pass
+constexpr int kMsdL = 8;               // lags per lane and pass
 constexpr int kMsdPass = 64 * kMsdL;   // lags per pass
 constexpr int kMsdWin = 1024;          // path rows held in LDS
+constexpr int kMsdRows = kMsdWin + kMsdWin / 8;  // with the pads
+// LDS position of window row r
+__device__ inline int msd_row(int r) { return r + (r >> 3); }
 constexpr int kMsdXcds = 8;
 constexpr int kMsdWaves = 4;           // wavefronts per unit
 constexpr int kMsdThreads = 64 * kMsdWaves;
@@ -1181,10 +1186,11 @@ msd_unit_kernel(const double* __restrict__ rows,
                 const int32_t* __restrict__ units,
                 double* __restrict__ scratch) {
   // the window; after a pass its space takes the wavefronts' lag sums
-  __shared__ double W[3 * kMsdWin];
+  __shared__ double W[3 * kMsdRows];
+  static_assert(kMsdWaves * kMsdPass <= 3 * kMsdRows, "lag sums fit");
   double* X = W;
-  double* Y = W + kMsdWin;
-  double* Z = W + 2 * kMsdWin;
+  double* Y = W + kMsdRows;
+  double* Z = W + 2 * kMsdRows;
   const int32_t u = units[blockIdx.x];
   if (u < 0) return;  // padding of an XCD's list (the whole workgroup)
   const MsdTraj d = tr[u >> 6];
@@ -1217,16 +1223,18 @@ msd_unit_kernel(const double* __restrict__ rows,
       __syncthreads();
       for (int r = tid; r < nw; r += kMsdThreads) {
         const int64_t o = (wlo + r) * w3;
-        X[wofs + r] = src[o];
-        Y[wofs + r] = src[o + 1];
-        Z[wofs + r] = src[o + 2];
+        const int pr = msd_row(wofs + r);
+        X[pr] = src[o];
+        Y[pr] = src[o + 1];
+        Z[pr] = src[o + 2];
       }
       if (!first)
         for (int r = tid; r < T1 - T0; r += kMsdThreads) {
           const int64_t o = (T0 + r) * w3;
-          X[r] = src[o];
-          Y[r] = src[o + 1];
-          Z[r] = src[o + 2];
+          const int pr = msd_row(r);
+          X[pr] = src[o];
+          Y[pr] = src[o + 1];
+          Z[pr] = src[o + 2];
         }
       __syncthreads();
       // wavefront `wave` takes the origins [ta, tb) of the segment
@@ -1239,7 +1247,7 @@ msd_unit_kernel(const double* __restrict__ rows,
       const int rel = int(lo - pass0) + wofs;  // LDS row of r(T0 + lo)
       double wx[kMsdL + 1], wy[kMsdL + 1], wz[kMsdL + 1];
       auto ld = [&](int row, double& x, double& y, double& z) {
-        const int rr = row < kMsdWin ? row : kMsdWin - 1;
+        const int rr = msd_row(row < kMsdWin ? row : kMsdWin - 1);
         x = X[rr];
         y = Y[rr];
         z = Z[rr];
@@ -1263,7 +1271,8 @@ msd_unit_kernel(const double* __restrict__ rows,
           const int oi = int(t0 - T0) + j;
           ld(rel + oi + kMsdL, wx[(j + kMsdL) % kR], wy[(j + kMsdL) % kR],
              wz[(j + kMsdL) % kR]);
-          const double ox = X[oi], oy = Y[oi], oz = Z[oi];
+          const int po = msd_row(oi);
+          const double ox = X[po], oy = Y[po], oz = Z[po];
 #pragma unroll
           for (int i = 0; i < kMsdL; ++i) {
             const int qq = (j + i) % kR;
@@ -1281,7 +1290,8 @@ msd_unit_kernel(const double* __restrict__ rows,
           const int oi = int(t0 - T0) + j;
           ld(rel + oi + kMsdL, wx[(j + kMsdL) % kR], wy[(j + kMsdL) % kR],
              wz[(j + kMsdL) % kR]);
-          const double ox = X[oi], oy = Y[oi], oz = Z[oi];
+          const int po = msd_row(oi);
+          const double ox = X[po], oy = Y[po], oz = Z[po];
           const int64_t nv = P - (t0 + j + lo);  // lags i < nv have rows
 #pragma unroll
           for (int i = 0; i < kMsdL; ++i) {
