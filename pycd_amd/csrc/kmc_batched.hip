@@ -15,10 +15,12 @@
 // slots may differ from cell 0's (neighbour lists are sorted by site index,
 // which wraps at the supercell boundary); one word per site holds that
 // permutation (3 bits per slot), so every site keeps the slot order, and
-// with it the process order of the selection, that its own table gives.  The compact rows (0.6 MB for an 8x8x8
-// corundum supercell against 302 MB for the full table) stay in L2, so the
-// per-hop gathers are cache hits rather than HBM round trips.  Sites are
-// carried packed: cx | cy << 8 | cz << 16 | sub << 24.
+// with it the process order of the selection, that its own table gives.
+// The compact rows and the hop differences (0.6 + 2.4 MB for an 8x8x8
+// corundum supercell against 302 MB for the full table) stay in the L2 of
+// the XCDs the family's trajectories are dealt to, so the per-hop gathers
+// are cache hits rather than HBM round trips.  Sites are carried packed:
+// cx | cz << 8 | cy << 16 | sub << 24.
 //
 // Scheduling: one wavefront (64 lanes, one workgroup) owns one trajectory at
 // a time and takes the next from a global atomic queue when it finishes, in
@@ -27,7 +29,8 @@
 //
 // Per-hop algebra is the plain path's (q.A[x,:] = phi_L[x] + psi[x]), but
 // psi is maintained incrementally: when carrier m hops s -> n, every slot
-// value changes by q (A[n, x] - A[s, x]) (two gathers per slot) and carrier
+// value changes by q (A[n, x] - A[s, x]), one gather per slot from the
+// family's hop differences (two table gathers without them), and carrier
 // m's new slots are summed from scratch (n_c gathers each); every `refresh`
 // hops all of psi is recomputed from the table to bound the rounding.
 //
@@ -611,11 +614,9 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
     const bool field_on =
         td.field[0] != 0.0 || td.field[1] != 0.0 || td.field[2] != 0.0;
 
-    // resumable per-trajectory state in global memory.  Displacement and
-    // drift live there while the trajectory runs: only this wavefront
-    // reads and writes them, lanes 0-2 read the mover's in the hop's round
-    // of loads and store the sums (plain stores; an atomic add per hop
-    // would be carried to memory each time).
+    // resumable per-trajectory state in global memory (the displacement
+    // and drift there are written with the state and read on resumption;
+    // while the trajectory runs they are held in the carriers' lanes)
     uint32_t* mts = a.mts + int64_t(tr) * 625;
     uint32_t* socc = a.st_occ + int64_t(tr) * kMaxCarriers;
     double* svec = a.st_vec + int64_t(tr) * kVecStride;
