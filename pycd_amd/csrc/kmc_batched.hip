@@ -1136,21 +1136,27 @@ kmc_pool_kernel(BatchArgs a) {
 //
 // msd_unit_kernel: one workgroup of kMsdWaves wavefronts per (trajectory,
 // carrier) unit.  The carrier's positions pass through LDS (x, y and z
-// arrays) in windows read from global memory once each; the wavefronts
-// take consecutive quarters of a window's origins and their lag sums are
-// added in wavefront order at the end of a pass (so that one window serves
-// four wavefronts and a CU holds 16 of them).  A pass takes kMsdPass lags,
-// lane l the kMsdL consecutive lags tau0 + kMsdL l + i; the lane walks its
-// origins t0 with r(t0 + kMsdL l + i), i = 0 .. kMsdL, in a ring of
-// registers, so a step reads the origin (one address for the whole
-// wavefront) and one new position, and forms kMsdL squared distances.  Rows
-// are held with one pad double after every 8, so the rows of lanes l and
-// l + 1 (8 apart) sit 9 doubles apart and the 16 lanes of an LDS access
-// fall on distinct banks; kMsdL = 8 leaves one lane idle for 500 lags.
-// For the first pass the window holds rows [T0, T0 + S + span - 1), which
-// contain the segment's S origins; later passes keep the origins in their
-// own S rows ahead of the window.  Sums over origins, unscaled, per (unit,
-// lag) to scratch.
+// planes) in windows read from global memory once each.  A pass takes
+// kMsdPass lags in blocks of kMsdBlock; in block b lane l owns the kMsdL
+// consecutive lags pass0 + kMsdBlock b + kMsdL l + i.  For a block, a
+// wavefront walks consecutive origins t0 and keeps r(t0 + lag) of its lane's
+// lags in a ring of kMsdRing registers, so a step reads the origin (one
+// address for the whole wavefront) and one new position, and forms kMsdL
+// squared distances.  Origin t0 has work in block b only while
+// t0 < P - pass0 - kMsdBlock b, so the blocks of high lags have fewer
+// origins; the segment's (block, origin) steps are cut into kMsdWaves equal
+// runs, one per wavefront (a run may end one block and start the next), and
+// the steps near the end of the written rows, where some of a lane's lags
+// fall past them, mask those lags.
+// Rows are held with one pad double after every 4, so the rows of lanes l
+// and l + 1 (4 apart) sit 5 doubles apart and the lanes of an LDS access
+// fall on distinct banks.  Runs start at multiples of 4 rows, so within a
+// run every LDS address is a per-lane (or per-wavefront) base plus an
+// immediate offset.  For the first pass the window holds rows
+// [T0, T0 + S + span - 1), which contain the segment's S origins; later
+// passes keep the origins in their own S rows ahead of the window.  Sums
+// over origins, unscaled, per (unit, lag) to scratch: each wavefront's sums
+// added in wavefront order.
 //
 // msd_reduce_kernel: one thread per (trajectory, lag) sums the carriers in
 // order and scales by scale^2 / (P_t - tau) / n_c: the same order in every
@@ -1169,15 +1175,110 @@ struct MsdTraj {
   int64_t sbase;    // scratch offset of carrier 0's M lag sums
 };
 
-constexpr int kMsdL = 8;               // lags per lane and pass
-constexpr int kMsdPass = 64 * kMsdL;   // lags per pass
-constexpr int kMsdWin = 1024;          // path rows held in LDS
-constexpr int kMsdRows = kMsdWin + kMsdWin / 8;  // with the pads
+constexpr int kMsdL = 4;                       // lags per lane and block
+constexpr int kMsdBlock = 64 * kMsdL;          // lags per block
+constexpr int kMsdBlocks = 2;                  // blocks per pass
+constexpr int kMsdPass = kMsdBlocks * kMsdBlock;  // lags per pass
+constexpr int kMsdRing = 8;                    // ring registers (> kMsdL)
+constexpr int kMsdWin = 1024;                  // path rows held in LDS
+constexpr int kMsdPlane = kMsdWin + kMsdWin / 4;  // a plane with its pads
 // LDS position of window row r
-__device__ inline int msd_row(int r) { return r + (r >> 3); }
+__device__ inline int msd_row(int r) { return r + (r >> 2); }
 constexpr int kMsdXcds = 8;
 constexpr int kMsdWaves = 4;           // wavefronts per unit
 constexpr int kMsdThreads = 64 * kMsdWaves;
+static_assert(kMsdRing % 4 == 0, "a ring turn keeps the runs' row phase");
+
+// One wavefront's run over the relative origins [ra, rb) of a block (ra a
+// multiple of 4).  L: the window, window row r at L[3 msd_row(r) + k] for
+// coordinate k; lo: the lane's first lag relative to the pass; hi: the
+// block's last lag below the pass's span; lim: P - pass0 - T0, the relative
+// origin bound; base: window row of relative origin 0's lag-0 row.  It reads
+// the rows below base + rb + lo + kMsdRing and the origins below rb + 2.
+__device__ inline void msd_run(const double* L, int ra, int rb, int lo,
+                               int hi, int lim, int base, double* acc) {
+  constexpr int kR = kMsdRing;
+  // rows are read at vb + immediate and origins at ob + immediate, both at
+  // multiples of 4 rows: msd_row(4a + p) = 5a + p + (p >> 2)
+  int vb = 3 * (((base + ra + lo) >> 2) * 5);
+  int ob = 3 * ((ra >> 2) * 5);
+  auto off = [](int p) { return 3 * (p + (p >> 2)); };
+  double wx[kR], wy[kR], wz[kR], ox[2], oy[2], oz[2];
+#pragma unroll
+  for (int p = 0; p < kR; ++p) {
+    wx[p] = L[vb + off(p)];
+    wy[p] = L[vb + off(p) + 1];
+    wz[p] = L[vb + off(p) + 2];
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    ox[p] = L[ob + off(p)];
+    oy[p] = L[ob + off(p) + 1];
+    oz[p] = L[ob + off(p) + 2];
+  }
+  // below fast_end every lane's lags up to hi have rows inside the written
+  // rows
+  const int fast_end = min(rb, lim - hi);
+  int o = ra;
+  // at step j of a turn slot (p % kR) holds r(o + lo + p) for
+  // p = j .. j + kR - 1 and origin slot (j % 2) holds r(o + j); after its
+  // pairs the step loads position j + kR and origin j + 2 into the slots
+  // that position j and origin j left (the scheduling barrier keeps those
+  // loads after the pairs, so a slot keeps its registers)
+  for (; o + kR <= fast_end; o += kR) {
+#pragma unroll
+    for (int j = 0; j < kR; ++j) {
+#pragma unroll
+      for (int i = 0; i < kMsdL; ++i) {
+        const int q = (j + i) % kR;
+        const double dx = wx[q] - ox[j & 1], dy = wy[q] - oy[j & 1],
+                     dz = wz[q] - oz[j & 1];
+        acc[i] = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[i])));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      wx[j] = L[vb + off(j + kR)];
+      wy[j] = L[vb + off(j + kR) + 1];
+      wz[j] = L[vb + off(j + kR) + 2];
+      ox[j & 1] = L[ob + off(j + 2)];
+      oy[j & 1] = L[ob + off(j + 2) + 1];
+      oz[j & 1] = L[ob + off(j + 2) + 2];
+    }
+    vb += 3 * (kR / 4 * 5);
+    ob += 3 * (kR / 4 * 5);
+  }
+  // masked steps: pair i of step j counts while lo + i < lim - o - j, for
+  // each pair a first part of the lanes (an exec mask, not a select)
+  auto masked_step = [&](int j) {
+    const int c = lim - (o + j);
+#pragma unroll
+    for (int i = 0; i < kMsdL; ++i) {
+      const int q = (j + i) % kR;
+      const double dx = wx[q] - ox[j & 1], dy = wy[q] - oy[j & 1],
+                   dz = wz[q] - oz[j & 1];
+      if (lo + i < c)
+        acc[i] = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[i])));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    wx[j] = L[vb + off(j + kR)];
+    wy[j] = L[vb + off(j + kR) + 1];
+    wz[j] = L[vb + off(j + kR) + 2];
+    ox[j & 1] = L[ob + off(j + 2)];
+    oy[j & 1] = L[ob + off(j + 2) + 1];
+    oz[j & 1] = L[ob + off(j + 2) + 2];
+  };
+  // whole turns, then the last part of a turn
+  for (; o + kR <= rb; o += kR) {
+#pragma unroll
+    for (int j = 0; j < kR; ++j) masked_step(j);
+    vb += 3 * (kR / 4 * 5);
+    ob += 3 * (kR / 4 * 5);
+  }
+#pragma unroll
+  for (int j = 0; j < kR; ++j) {
+    if (o + j >= rb) break;
+    masked_step(j);
+  }
+}
 
 __global__ void __launch_bounds__(kMsdThreads)
 __attribute__((amdgpu_waves_per_eu(4, 4)))
@@ -1186,18 +1287,17 @@ msd_unit_kernel(const double* __restrict__ rows,
                 const TrajOut* __restrict__ outs,
                 const int32_t* __restrict__ units,
                 double* __restrict__ scratch) {
-  // the window; after a pass its space takes the wavefronts' lag sums
-  __shared__ double W[3 * kMsdRows];
-  static_assert(kMsdWaves * kMsdPass <= 3 * kMsdRows, "lag sums fit");
-  double* X = W;
-  double* Y = W + kMsdRows;
-  double* Z = W + 2 * kMsdRows;
+  // the window (x, y, z of a row side by side); after a pass its space
+  // takes the wavefronts' lag sums
+  __shared__ double L[3 * kMsdPlane];
+  static_assert(kMsdWaves * kMsdPass <= 3 * kMsdPlane, "lag sums fit");
   const int32_t u = units[blockIdx.x];
   if (u < 0) return;  // padding of an XCD's list (the whole workgroup)
   const MsdTraj d = tr[u >> 6];
   const int c = u & 63;
   const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
   // only the rows written by the run: [0, start) (all P when the path filled)
   const int64_t written = outs[d.traj].start;
   const int64_t P = written < 1 ? 1 : (written < d.P ? written : d.P);
@@ -1206,14 +1306,19 @@ msd_unit_kernel(const double* __restrict__ rows,
   double* out = scratch + d.sbase + int64_t(c) * d.M;
   for (int64_t pass0 = 0; pass0 < d.M; pass0 += kMsdPass) {
     const int span = int(min<int64_t>(kMsdPass, d.M - pass0));
-    const int64_t lo = pass0 + int64_t(kMsdL) * lane;  // this lane's lags
-    double acc[kMsdL];
+    const int nblk = (span + kMsdBlock - 1) / kMsdBlock;
+    double acc[kMsdBlocks][kMsdL];
 #pragma unroll
-    for (int i = 0; i < kMsdL; ++i) acc[i] = 0.0;
-    // origins per segment, and where the window rows sit in LDS
+    for (int b = 0; b < kMsdBlocks; ++b)
+#pragma unroll
+      for (int i = 0; i < kMsdL; ++i) acc[b][i] = 0.0;
+    // origins per segment (a multiple of 4), and where the window rows sit
+    // in LDS: a run reads rows below wofs + S + kMsdBlock nblk + 4, inside
+    // the window
     const bool first = pass0 == 0;
-    const int S = first ? kMsdWin - span + 1 : (kMsdWin - span + 1) / 2;
-    const int wofs = first ? 0 : S;  // LDS row of window row T0 + pass0
+    const int room = kMsdWin - kMsdBlock * nblk - 4;
+    const int S = (first ? room : room / 2) & ~3;
+    const int wofs = first ? 0 : S;  // window row of path row T0 + pass0
     // origins t0 < P - pass0 have a lag of this pass inside the rows
     const int64_t t_end = P - pass0;
     for (int64_t T0 = 0; T0 < t_end; T0 += S) {
@@ -1224,95 +1329,54 @@ msd_unit_kernel(const double* __restrict__ rows,
       __syncthreads();
       for (int r = tid; r < nw; r += kMsdThreads) {
         const int64_t o = (wlo + r) * w3;
-        const int pr = msd_row(wofs + r);
-        X[pr] = src[o];
-        Y[pr] = src[o + 1];
-        Z[pr] = src[o + 2];
+        const int pr = 3 * msd_row(wofs + r);
+        L[pr] = src[o];
+        L[pr + 1] = src[o + 1];
+        L[pr + 2] = src[o + 2];
       }
       if (!first)
         for (int r = tid; r < T1 - T0; r += kMsdThreads) {
           const int64_t o = (T0 + r) * w3;
-          const int pr = msd_row(r);
-          X[pr] = src[o];
-          Y[pr] = src[o + 1];
-          Z[pr] = src[o + 2];
+          const int pr = 3 * msd_row(r);
+          L[pr] = src[o];
+          L[pr + 1] = src[o + 1];
+          L[pr + 2] = src[o + 2];
         }
       __syncthreads();
-      // wavefront `wave` takes the origins [ta, tb) of the segment
-      const int64_t q = (T1 - T0 + kMsdWaves - 1) / kMsdWaves;
-      const int64_t ta = min<int64_t>(T1, T0 + wave * q);
-      const int64_t tb = min<int64_t>(T1, ta + q);
-      // every lane of a wavefront walks the same origins (its ring slots
-      // line up with the others'); a lag whose row lies past the written
-      // rows, or past M, adds nothing (lags past M are dropped at the end)
-      const int rel = int(lo - pass0) + wofs;  // LDS row of r(T0 + lo)
-      double wx[kMsdL + 1], wy[kMsdL + 1], wz[kMsdL + 1];
-      auto ld = [&](int row, double& x, double& y, double& z) {
-        const int rr = msd_row(row < kMsdWin ? row : kMsdWin - 1);
-        x = X[rr];
-        y = Y[rr];
-        z = Z[rr];
-      };
-      const int base = rel + int(ta - T0);  // LDS row of r(ta + lo)
+      // block b has the relative origins [0, n_b); the blocks' steps, each
+      // count rounded up to a multiple of 4, are cut into kMsdWaves runs
+      const int lim = int(min<int64_t>(t_end - T0, int64_t(1) << 30));
+      const int seg = int(T1 - T0);
+      int n[kMsdBlocks], tot = 0;
 #pragma unroll
-      for (int i = 0; i < kMsdL; ++i) ld(base + i, wx[i], wy[i], wz[i]);
-      // the highest first lag among lanes with a lag inside the pass and the
-      // written rows: below fast_end every such lane has all its rows
-      const int64_t reach = min<int64_t>(span, P - pass0);
-      const int64_t lo_top = pass0 + kMsdL * ((reach - 1) / kMsdL);
-      const int64_t fast_end = min<int64_t>(tb, P - (lo_top + kMsdL - 1));
-      constexpr int kR = kMsdL + 1;  // ring slots; a turn is kR origins
-      int64_t t0 = ta;
-      // Whole turns: at step j of a turn slot (p % kR) holds
-      // r(t0 + lo + p) for p = j .. j + kMsdL - 1; the step loads position
-      // j + kMsdL into the slot position j - 1 left
-      for (; t0 + kR <= fast_end; t0 += kR) {
-#pragma unroll
-        for (int j = 0; j < kR; ++j) {
-          const int oi = int(t0 - T0) + j;
-          ld(rel + oi + kMsdL, wx[(j + kMsdL) % kR], wy[(j + kMsdL) % kR],
-             wz[(j + kMsdL) % kR]);
-          const int po = msd_row(oi);
-          const double ox = X[po], oy = Y[po], oz = Z[po];
-#pragma unroll
-          for (int i = 0; i < kMsdL; ++i) {
-            const int qq = (j + i) % kR;
-            const double dx = wx[qq] - ox, dy = wy[qq] - oy, dz = wz[qq] - oz;
-            acc[i] = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[i])));
-          }
-        }
+      for (int b = 0; b < kMsdBlocks; ++b) {
+        n[b] = b < nblk ? max(0, min(seg, lim - kMsdBlock * b)) : 0;
+        tot += (n[b] + 3) & ~3;
       }
-      // the rest: lags checked against the written rows, origins against
-      // the chunk's end
-      for (; t0 < tb; t0 += kR) {
+      const int Q = ((tot + kMsdWaves - 1) / kMsdWaves + 3) & ~3;
+      const int a = wave * Q, e = min(a + Q, tot);
+      int boff = 0;
 #pragma unroll
-        for (int j = 0; j < kR; ++j) {
-          if (t0 + j >= tb) break;
-          const int oi = int(t0 - T0) + j;
-          ld(rel + oi + kMsdL, wx[(j + kMsdL) % kR], wy[(j + kMsdL) % kR],
-             wz[(j + kMsdL) % kR]);
-          const int po = msd_row(oi);
-          const double ox = X[po], oy = Y[po], oz = Z[po];
-          const int64_t nv = P - (t0 + j + lo);  // lags i < nv have rows
-#pragma unroll
-          for (int i = 0; i < kMsdL; ++i) {
-            const int qq = (j + i) % kR;
-            const double dx = wx[qq] - ox, dy = wy[qq] - oy, dz = wz[qq] - oz;
-            const double a = fma(dz, dz, fma(dy, dy, fma(dx, dx, acc[i])));
-            acc[i] = i < nv ? a : acc[i];
-          }
-        }
+      for (int b = 0; b < kMsdBlocks; ++b) {
+        const int ra = max(a - boff, 0), rb = min(e - boff, n[b]);
+        if (ra < rb)
+          msd_run(L, ra, rb, kMsdBlock * b + kMsdL * lane,
+                  min(span, kMsdBlock * (b + 1)) - 1, lim, wofs, acc[b]);
+        boff += (n[b] + 3) & ~3;
       }
     }
     // the wavefronts' sums of each lag added in wavefront order
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kMsdL; ++i) W[wave * kMsdPass + kMsdL * lane + i] = acc[i];
+    for (int b = 0; b < kMsdBlocks; ++b)
+#pragma unroll
+      for (int i = 0; i < kMsdL; ++i)
+        L[wave * kMsdPass + kMsdBlock * b + kMsdL * lane + i] = acc[b][i];
     __syncthreads();
     for (int l = tid; l < span; l += kMsdThreads) {
-      double v = W[l];
+      double v = L[l];
 #pragma unroll
-      for (int w = 1; w < kMsdWaves; ++w) v += W[w * kMsdPass + l];
+      for (int w = 1; w < kMsdWaves; ++w) v += L[w * kMsdPass + l];
       out[pass0 + l] = v;
     }
   }

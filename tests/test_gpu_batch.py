@@ -284,10 +284,10 @@ def test_fp64_batch_monoclinic_matches_oracle_per_trajectory():
 
 
 def test_msd_lag_passes_and_many_carriers():
-    """The MSD kernels take lags in passes of 576 and one wavefront per
-    (trajectory, carrier): lags reaching into the second and third pass,
-    and 64, 33 and 2 electrons, against the oracle's squared displacements
-    (each path filled: 2001 rows)."""
+    """The MSD kernels take lags in passes of 512 (two blocks of 256) and
+    one workgroup per (trajectory, carrier): lags reaching into the second
+    and third pass, and 64, 33 and 2 electrons, against the oracle's squared
+    displacements (each path filled: 2001 rows)."""
     fams, runs = _families([4], t_final=2e-8)
     specs = [(0, 64, 0.0, 300.0), (0, 2, 1e6, 350.0), (0, 33, 0.0, 280.0)]
     seed = 23
@@ -311,4 +311,47 @@ def test_msd_lag_passes_and_many_carriers():
         ref = omsd.squared_displacement(pos, 1, P, lags[k])[0].mean(axis=1)
         assert got[k][0] == 0.0
         assert np.allclose(got[k], ref, rtol=1e-11, atol=1e-300), k
+    bt.close()
+
+
+def test_msd_partly_written_paths():
+    """Paths the runs fill only in part (151, 420, 918, 3544 and 312 of
+    4001 rows: the rate estimate that sets the time grid counts no blocked
+    hops, so the 8-electron run writes more rows than its target), with lags
+    past the written rows, one and three passes of lags and 1-8 electrons:
+    the lags below the written rows against the oracle's squared
+    displacements, the rest zero."""
+    fams, runs = _families([4], t_final=2e-8)
+    specs = [(0, 1, 0.0, 300.0), (0, 3, 1e6, 350.0), (0, 5, 0.0, 280.0),
+             (0, 8, 0.0, 320.0), (0, 2, 0.0, 300.0)]
+    targets = [150, 400, 700, 1500, 300]
+    lags = [500, 500, 500, 1100, 1200]
+    hops = 6000
+    trajs, occs, words = _mixed_batch(fams, runs, specs, 29, hops)
+    for t, (f, nc, fv, T), p in zip(trajs, specs, targets):
+        rate = B.expected_rate_per_carrier(fams[f], t["kT"], t["field"]) * nc
+        t["time_interval"] = hops / rate / p
+        t["n_path"] = 4001
+    scale = 1 / constants.ANG2BOHR
+    bt = B.KmcBatch(fams, "f64", "f32")
+    bt.setup(trajs, occs, words, msd_lags=lags, msd_scale=scale,
+             waves_per_cu=1)
+    bt.run(use_graph=False)
+    res = bt.results()
+    got = bt.msd()
+    seen = []
+    for k, (f, nc, fv, T) in enumerate(specs):
+        P = int(res["start"][k])
+        assert 1 < P < 4001, (k, P, list(res["start"]))
+        seen.append(P)
+        m = min(P, lags[k])
+        pos = bt.rows(k)[:P].reshape(P, nc, 3) * scale
+        ref = omsd.squared_displacement(pos, 1, P, m)[0].mean(axis=1)
+        assert got[k][0] == 0.0
+        assert np.allclose(got[k][:m], ref, rtol=1e-11, atol=1e-300), k
+        assert np.all(got[k][m:] == 0.0), k
+    # the written rows fall below one block of lags, inside the second block
+    # and past the pass
+    assert min(seen) < 256 and any(256 < P < 512 for P in seen)
+    assert max(seen) > 1100
     bt.close()
