@@ -109,9 +109,12 @@ int pycd_pair_table(pycd_ctx* ctx, const double* pos, int64_t n,
  * the variants and sets the fastest.  Every variant gives the same table
  * bit for bit.
  *   PYCD_PREC_F64     1 = 64 x 64 tiles, 2 = 128 x 128 tiles
- *   PYCD_PREC_BF16X3  1 = K staged in slices of 32, 2 = slices of 64
+ *   PYCD_PREC_BF16X3  1 = K staged in slices of 32, 2 = slices of 64,
+ *                     3 = slices of 32 in two LDS stages, 4 = 256 x 256
+ *                     tiles (8 wavefronts) in two LDS stages
  *   PYCD_PREC_F32     1 (the only one)
- *   0 = the default (FP64: 128 x 128 from 2048 sites on; BF16x3: 64). */
+ *   0 = the default (FP64: 128 x 128 from 2048 sites on; BF16x3: 4 from
+ *   4096 sites on, 2 below). */
 int pycd_set_table_variant(pycd_ctx* ctx, int32_t precision, int32_t variant);
 /* The variant set for `precision` (0 = default), or -1 for a bad call. */
 int pycd_table_variant(const pycd_ctx* ctx, int32_t precision);

@@ -227,8 +227,9 @@ def pair_table(pos, cell, trans, pbc, alpha, r_cut, k_cut, eps_r,
 def set_table_variant(precision, variant):
     """Tile variant of the table kernels for `precision` in this process's
     context (0 = default; FP64: 1 = 64 x 64 tiles, 2 = 128 x 128; BF16x3:
-    1 = K slices of 32, 2 = of 64; FP32: 1).  Every variant gives the same
-    table bit for bit."""
+    1 = K slices of 32, 2 = of 64, 3 = of 32 in two LDS stages, 4 = 256 x 256
+    tiles in two LDS stages; FP32: 1).  Every variant gives the same table
+    bit for bit."""
     ctx = context()
     _check(ctx, lib().pycd_set_table_variant(ctx, PREC[precision],
                                              int(variant)),
@@ -240,7 +241,7 @@ def table_variant(precision):
     return int(lib().pycd_table_variant(context(), PREC[precision]))
 
 
-TABLE_VARIANTS = {"f64": (1, 2), "f32": (1,), "bf16x3": (1, 2)}
+TABLE_VARIANTS = {"f64": (1, 2), "f32": (1,), "bf16x3": (1, 2, 3, 4)}
 
 
 class DeviceArray:

@@ -86,8 +86,10 @@ extern "C" int pycd_set_table_variant(pycd_ctx* ctx, int precision,
   if (precision < PYCD_PREC_F64 || precision > PYCD_PREC_BF16X3)
     return pycd::fail(ctx, PYCD_ERR_ARG, "unknown precision %d", precision);
   // FP64: 1 = 64 x 64 tiles, 2 = 128 x 128; BF16x3: 1 = K slices of 32,
-  // 2 = slices of 64; FP32: 1 only; 0 = default
-  const int hi = precision == PYCD_PREC_F32 ? 1 : 2;
+  // 2 = slices of 64, 3 = slices of 32 in two LDS stages, 4 = 256 x 256
+  // tiles in two LDS stages; FP32: 1 only; 0 = default
+  const int hi = precision == PYCD_PREC_F32 ? 1
+                 : precision == PYCD_PREC_BF16X3 ? 4 : 2;
   if (variant < 0 || variant > hi)
     return pycd::fail(ctx, PYCD_ERR_ARG, "variant %d outside 0..%d", variant,
                       hi);

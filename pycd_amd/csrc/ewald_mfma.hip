@@ -13,7 +13,10 @@
 // Tiling: 256 threads = 4 wavefronts per 128 x 128 output tile; each
 // wavefront owns 64 x 64 = 4 x 4 MFMA blocks of 16 x 16.  K is staged
 // through LDS in slices of 32 (row-major, padded rows); the global loads of
-// the next slice are in flight while the current one is multiplied.
+// the next slice are in flight while the current one is multiplied.  The
+// BF16X3 mode also has two-stage kernels (one barrier per slice, loads two
+// or three slices ahead) on 128 x 128 tiles and on 256 x 256 tiles of 8
+// wavefronts; the latter is the default from 4096 sites on.
 #include "ewald_common.hpp"
 
 #include <hip/hip_bf16.h>
@@ -272,6 +275,276 @@ table_mfma_bf16x3(const __bf16* __restrict__ Yhi,
   count_pairs(cnt, pairs);
 }
 
+// K slices of 32 in two LDS stages and three register sets: while slice t
+// is multiplied out of one stage, slice t + 1 (in registers since two steps
+// before) goes to the other stage and slice t + 3 is loaded into the
+// registers slice t left, so a global load has three steps to arrive.  One barrier per slice, without a
+// memory fence: it waits for this wavefront's LDS operations only, and the
+// loads in flight stay in flight.  Two workgroups fill a CU's 160 KB of LDS.
+// The sums over k run in the same order as the one-stage kernels, so the
+// table is the same bit for bit.  Y holds whole 128-row tiles (rows past n
+// are zero) and Kp is a multiple of 6 kB (zero columns), so every load and
+// LDS store is made unconditionally: a branch around them would make the
+// compiler wait for all loads in flight.
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(2, 2)))
+table_mfma_bf16x3_2stage(const __bf16* __restrict__ Yhi,
+                         const __bf16* __restrict__ Ylo, int n, int Kp,
+                         const double* __restrict__ pos, MicGeometry g,
+                         EwaldScalars es, double* __restrict__ A,
+                         unsigned long long* __restrict__ pairs) {
+  constexpr int kB = 32;
+  constexpr int kRow = kB + 8;  // padded row (bf16 elements)
+  constexpr int kArr = kMTile * kRow;
+  // stage s, array a (hi A, lo A, hi B, lo B) at S + (4 s + a) kArr
+  __shared__ __attribute__((aligned(16))) __bf16 S[2 * 4 * kArr];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  int bi, bj;
+  tile_coords_upper(n, kMTile, bi, bj);
+  const int i0 = bi * kMTile, j0 = bj * kMTile;
+  const bool mirror = bi != bj;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  v4f acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  // loader: 128 rows x 32 bf16 per array = 256 threads x one 16-byte chunk
+  // of each of hi/lo at two k offsets
+  constexpr int kQ = kB / 16;
+  const int lrow = t >> 1, lk = (t & 1) * (kB / 2);
+  const int64_t aoff = int64_t(i0 + lrow) * Kp + lk;
+  const int64_t boff = int64_t(j0 + lrow) * Kp + lk;
+  struct Regs { uint4 v[4][kQ]; };
+  auto gload = [&](Regs& r, int k0) {
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      r.v[0][q] = load16_global(Yhi + aoff + k0 + 8 * q);
+      r.v[1][q] = load16_global(Ylo + aoff + k0 + 8 * q);
+      r.v[2][q] = load16_global(Yhi + boff + k0 + 8 * q);
+      r.v[3][q] = load16_global(Ylo + boff + k0 + 8 * q);
+    }
+  };
+  auto sstore = [&](int stage, const Regs& r) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int q = 0; q < kQ; ++q)
+        *reinterpret_cast<uint4*>(S + (4 * stage + a) * kArr + lrow * kRow +
+                                  lk + 8 * q) = r.v[a][q];
+  };
+  // all of this wavefront's LDS operations done, then the barrier
+  auto lds_barrier = [] {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  };
+  auto multiply = [&](int stage) {
+    const __bf16* Ah = S + (4 * stage + 0) * kArr;
+    const __bf16* Al = S + (4 * stage + 1) * kArr;
+    const __bf16* Bh = S + (4 * stage + 2) * kArr;
+    const __bf16* Bl = S + (4 * stage + 3) * kArr;
+    const int kof = (lane >> 4) * 8;
+    v8bf b_h[4], b_l[4];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int rb = (wc + 16 * y + (lane & 15)) * kRow + kof;
+      b_h[y] = *reinterpret_cast<const v8bf*>(Bh + rb);
+      b_l[y] = *reinterpret_cast<const v8bf*>(Bl + rb);
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int ra = (wr + 16 * x + (lane & 15)) * kRow + kof;
+      const v8bf a_h = *reinterpret_cast<const v8bf*>(Ah + ra);
+      const v8bf a_l = *reinterpret_cast<const v8bf*>(Al + ra);
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            a_l, b_h[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            a_h, b_l[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            a_h, b_h[y], acc[x][y], 0, 0, 0);
+      }
+    }
+  };
+  // step for slice k0 (in stage `stage`): `next` holds slice k0 + kB,
+  // `spare` takes slice k0 + 3 kB; at the end of K they load and store the
+  // last slice again (into a stage no step reads)
+  // (scheduling barriers keep the loads at the start of the step and the
+  // LDS stores at its end, which the scheduler would otherwise swap)
+  auto step = [&](int stage, int k0, Regs& next, Regs& spare) {
+    gload(spare, min(k0 + 3 * kB, Kp - kB));
+    __builtin_amdgcn_sched_barrier(0);
+    multiply(stage);
+    __builtin_amdgcn_sched_barrier(0);
+    sstore(stage ^ 1, next);
+    lds_barrier();
+  };
+  Regs r0, r1, r2;
+  gload(r0, 0);
+  sstore(0, r0);
+  gload(r1, kB);
+  gload(r2, 2 * kB);
+  lds_barrier();
+  for (int k0 = 0; k0 < Kp; k0 += 6 * kB) {
+    step(0, k0, r1, r0);
+    step(1, k0 + kB, r2, r1);
+    step(0, k0 + 2 * kB, r0, r2);
+    step(1, k0 + 3 * kB, r1, r0);
+    step(0, k0 + 4 * kB, r2, r1);
+    step(1, k0 + 5 * kB, r0, r2);
+  }
+  unsigned cnt = 0;
+  // entry (i, j) and, off the diagonal block, its mirror (j, i): the same
+  // number, formed once (store_pair)
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r;
+        const int j = j0 + wc + 16 * y + (lane & 15);
+        if (i < n && j < n)
+          store_pair(A, n, i, j, mirror, pos, g, es, double(acc[x][y][r]),
+                     cnt);
+      }
+  count_pairs(cnt, pairs);
+}
+
+// 256 x 256 output tiles on 8 wavefronts (512 threads), each wavefront
+// 128 x 64 = 8 x 4 MFMA blocks: a step loads half the bytes per product of
+// the 128 x 128 tiles, and a wavefront reads a quarter less LDS per
+// product.  K slices of 32 in two LDS stages (160 KB, one workgroup per CU):
+// a step loads the next slice into registers as it starts and stores it to
+// the other stage as it ends, with the step's 96 MFMAs per wavefront in
+// between (one register set: the accumulators take half the registers).  For the table to equal
+// the 128 x 128 kernels' bit for bit, every entry is formed with the same
+// operands in the same roles: in a diagonal 256-tile the lower-left 128 x 128
+// block is not stored, its upper mirror writes it (as a 128 x 128 tile off
+// the diagonal does), and the two diagonal 128-blocks store both (i, j) and
+// (j, i) as formed.  Y holds whole 256-row tiles and Kp is a multiple of
+// 2 kB, so loads and LDS stores are unconditional.
+constexpr int kMTile2 = 256;
+__global__ void __launch_bounds__(512)
+__attribute__((amdgpu_waves_per_eu(2, 2)))
+table_mfma_bf16x3_256(const __bf16* __restrict__ Yhi,
+                      const __bf16* __restrict__ Ylo, int n, int Kp,
+                      const double* __restrict__ pos, MicGeometry g,
+                      EwaldScalars es, double* __restrict__ A,
+                      unsigned long long* __restrict__ pairs) {
+  constexpr int kB = 32;
+  constexpr int kRow = kB + 8;  // padded row (bf16 elements)
+  constexpr int kArr = kMTile2 * kRow;
+  // stage s, array a (hi A, lo A, hi B, lo B) at S + (4 s + a) kArr
+  __shared__ __attribute__((aligned(16))) __bf16 S[2 * 4 * kArr];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  int bi, bj;
+  tile_coords_upper(n, kMTile2, bi, bj);
+  const int i0 = bi * kMTile2, j0 = bj * kMTile2;
+  const int wr = (wave >> 2) * 128, wc = (wave & 3) * 64;
+  // in a diagonal tile: the upper-right 128-block is stored with its mirror,
+  // the lower-left one not at all
+  const bool diag = bi == bj;
+  const bool mirror = !diag || (wr == 0 && wc >= 128);
+  const bool skip = diag && wr == 128 && wc < 128;
+  v4f acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  // loader: 256 rows x 32 bf16 per array = 512 threads x two 16-byte chunks
+  constexpr int kQ = kB / 16;
+  const int lrow = t >> 1, lk = (t & 1) * (kB / 2);
+  const int64_t aoff = int64_t(i0 + lrow) * Kp + lk;
+  const int64_t boff = int64_t(j0 + lrow) * Kp + lk;
+  struct Regs { uint4 v[4][kQ]; };
+  auto gload = [&](Regs& r, int k0) {
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      r.v[0][q] = load16_global(Yhi + aoff + k0 + 8 * q);
+      r.v[1][q] = load16_global(Ylo + aoff + k0 + 8 * q);
+      r.v[2][q] = load16_global(Yhi + boff + k0 + 8 * q);
+      r.v[3][q] = load16_global(Ylo + boff + k0 + 8 * q);
+    }
+  };
+  auto sstore = [&](int stage, const Regs& r) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int q = 0; q < kQ; ++q)
+        *reinterpret_cast<uint4*>(S + (4 * stage + a) * kArr + lrow * kRow +
+                                  lk + 8 * q) = r.v[a][q];
+  };
+  auto lds_barrier = [] {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  };
+  auto multiply = [&](int stage) {
+    const __bf16* Ah = S + (4 * stage + 0) * kArr;
+    const __bf16* Al = S + (4 * stage + 1) * kArr;
+    const __bf16* Bh = S + (4 * stage + 2) * kArr;
+    const __bf16* Bl = S + (4 * stage + 3) * kArr;
+    const int kof = (lane >> 4) * 8;
+    v8bf b_h[4], b_l[4];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int rb = (wc + 16 * y + (lane & 15)) * kRow + kof;
+      b_h[y] = *reinterpret_cast<const v8bf*>(Bh + rb);
+      b_l[y] = *reinterpret_cast<const v8bf*>(Bl + rb);
+    }
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+      const int ra = (wr + 16 * x + (lane & 15)) * kRow + kof;
+      const v8bf a_h = *reinterpret_cast<const v8bf*>(Ah + ra);
+      const v8bf a_l = *reinterpret_cast<const v8bf*>(Al + ra);
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            a_l, b_h[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            a_h, b_l[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            a_h, b_h[y], acc[x][y], 0, 0, 0);
+      }
+    }
+  };
+  // step for slice k0 (in stage `stage`): slice k0 + kB is loaded and
+  // goes to the other stage; the last step loads and stores the last slice
+  // again (into a stage no step reads); scheduling barriers keep the loads
+  // at the start of the step and the LDS stores at its end
+  Regs r;
+  auto step = [&](int stage, int k0) {
+    gload(r, min(k0 + kB, Kp - kB));
+    __builtin_amdgcn_sched_barrier(0);
+    multiply(stage);
+    __builtin_amdgcn_sched_barrier(0);
+    sstore(stage ^ 1, r);
+    lds_barrier();
+  };
+  gload(r, 0);
+  sstore(0, r);
+  lds_barrier();
+  for (int k0 = 0; k0 < Kp; k0 += 2 * kB) {
+    step(0, k0);
+    step(1, k0 + kB);
+  }
+  if (skip) return;
+  unsigned cnt = 0;
+#pragma unroll
+  for (int x = 0; x < 8; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r;
+        const int j = j0 + wc + 16 * y + (lane & 15);
+        if (i < n && j < n)
+          store_pair(A, n, i, j, mirror, pos, g, es, double(acc[x][y][r]),
+                     cnt);
+      }
+  count_pairs(cnt, pairs);
+}
+
 int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
                       const double* kv, const double* sw, int K,
                       const MicGeometry& geo, const EwaldScalars& es,
@@ -279,11 +552,22 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
                       hipEvent_t after_sf) {
   hipStream_t s = ctx->stream;
   // K padded to whole slices of the mode's kernel (zero columns add nothing)
-  const int bvar = ctx->table_variant[PYCD_PREC_BF16X3];
-  const int bslice = bvar == 1 ? 32 : 64;
-  const int slice = precision == PYCD_PREC_F32 ? kMSlice : bslice;
+  // BF16x3 default: 256 x 256 tiles from 4096 sites on, K slices of 64
+  // below (a few large tiles would leave most CUs idle)
+  int bvar = ctx->table_variant[PYCD_PREC_BF16X3];
+  if (bvar == 0) bvar = n >= 4096 ? 4 : 2;
+  const int bslice = bvar == 1 || bvar >= 3 ? 32 : 64;
+  // (the two-stage kernels take K in groups of six and of two 32-slices)
+  const int slice = precision == PYCD_PREC_F32 ? kMSlice
+                    : bvar == 3              ? 6 * bslice
+                    : bvar == 4              ? 2 * bslice
+                                             : bslice;
   const int Kp = std::max(slice, ((2 * K + slice - 1) / slice) * slice);
-  const size_t elems = size_t(n) * Kp;
+  // rows in whole tiles: the rows past n stay zero
+  const int tile = precision == PYCD_PREC_BF16X3 && bvar == 4 ? kMTile2
+                                                              : kMTile;
+  const int nb = (n + tile - 1) / tile;
+  const size_t elems = size_t(nb) * tile * Kp;
   DevBuf<float> y32;
   DevBuf<__bf16> yhi, ylo;
   if (precision == PYCD_PREC_F32) {
@@ -304,11 +588,16 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
     PYCD_HIP(ctx, hipGetLastError());
   }
   PYCD_HIP(ctx, hipEventRecord(after_sf, s));
-  const int nb = (n + kMTile - 1) / kMTile;
   const int tiles = nb * (nb + 1) / 2;
   if (precision == PYCD_PREC_F32)
     table_mfma_f32<<<tiles, 256, 0, s>>>(y32.ptr, n, Kp, pos, geo, es, out,
                                          pairs);
+  else if (bvar == 3)
+    table_mfma_bf16x3_2stage<<<tiles, 256, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp,
+                                                   pos, geo, es, out, pairs);
+  else if (bvar == 4)
+    table_mfma_bf16x3_256<<<tiles, 512, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, pos,
+                                                geo, es, out, pairs);
   else if (bslice == 32)
     table_mfma_bf16x3<32><<<tiles, 256, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, pos,
                                                 geo, es, out, pairs);

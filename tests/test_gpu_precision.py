@@ -143,8 +143,9 @@ def test_precision_configuration_10x10x10():
 
 def test_table_variants_agree_and_autotune_picks_one():
     """Each precision's tile variants (FP64: 64 x 64 and 128 x 128 tiles;
-    BF16x3: K slices of 32 and 64) give the same table bit for bit, and the
-    autotuner keeps the fastest of them in the context."""
+    BF16x3: K slices of 32 and 64, and of 32 in two LDS stages; 2592 sites,
+    not a whole number of 128-row tiles) give the same table bit for bit,
+    and the autotuner keeps the fastest of them in the context."""
     from pycd_amd import tuning
     w = workloads.corundum(6)
     pos = w.neighbors.bulk_sites.cell_coordinates
