@@ -241,7 +241,7 @@ int build_ewald_setup(pycd_ctx* ctx, const double* trans, const double* cell,
       }
   st.K = int(st.sw.size());
   // minimum-image geometry
-  fill_mic_geometry(cell, pbc, st.geo);
+  fill_mic_geometry(cell, pbc, st.geo, /*distance_only=*/true);
   return PYCD_OK;
 }
 
@@ -275,8 +275,12 @@ extern "C" int pycd_pair_table(pycd_ctx* ctx, const double* pos, int64_t n,
   EwaldScalars es;
   es.sqrt_alpha = std::sqrt(alpha);
   es.r_cut = r_cut;
+  // fl(r_cut^2) is within half an ulp of the exact square; 2^-50 above it
+  // is above the exact square, so r^2 >= r_cut2 gives sqrt(r^2) >= r_cut
+  es.r_cut2 = r_cut * r_cut * (1.0 + 0x1p-50);
   es.eps = eps_r;
   es.self_term = -1.0 * std::sqrt(alpha / M_PI) / eps_r;
+  es.unit_eps = eps_r == 1.0;
 
   DevBuf<double> d_pos, d_kv, d_sw, d_A;
   DevBuf<unsigned long long> d_pairs;

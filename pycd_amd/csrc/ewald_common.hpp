@@ -24,8 +24,10 @@ struct MicGeometry {
 struct EwaldScalars {
   double sqrt_alpha;
   double r_cut;
+  double r_cut2;     // a few ulp above r_cut^2: r^2 >= r_cut2 gives r >= r_cut
   double eps;
   double self_term;  // -sqrt(alpha/pi)/eps
+  int unit_eps;      // eps == 1, where x / eps == x
 };
 
 struct EwaldSetup {
@@ -37,8 +39,10 @@ struct EwaldSetup {
   MicGeometry geo;
 };
 
+// distance_only: the users need the minimum distance, not which image
+// gives it (the Ewald table), so offsets that can only tie are dropped too.
 inline void fill_mic_geometry(const double* cell, const int32_t* pbc,
-                              MicGeometry& g) {
+                              MicGeometry& g, bool distance_only = false) {
   for (int k = 0; k < 9; ++k) g.cell[k] = cell[k];
   const double* m = cell;
   const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) -
@@ -73,7 +77,7 @@ inline void fill_mic_geometry(const double* cell, const int32_t* pbc,
           // boundary of the wrapped cell), so dropping it leaves the minimum
           // distance unchanged (to rounding where the reference's strict
           // comparison would see an exact tie).  Hexagonal and monoclinic
-          // supercells keep 12 of the 26 offsets, orthogonal ones none.
+          // supercells keep 12 of the 26 offsets here, orthogonal ones none.
           double reach = 0.0;
           for (int k = 0; k < 3; ++k)
             reach += std::fabs(cell[3 * k] * v[0] + cell[3 * k + 1] * v[1] +
@@ -84,6 +88,40 @@ inline void fill_mic_geometry(const double* cell, const int32_t* pbc,
         for (int c = 0; c < 3; ++c) g.dS[g.n_delta][c] = v[c];
         ++g.n_delta;
       }
+  if (all_pbc && distance_only) {
+    // An offset v is also never needed when another kept offset u is at
+    // least as short for every wrapped b: |b + v|^2 - |b + u|^2 =
+    // 2 b.(v - u) + |v|^2 - |u|^2, whose minimum over b is
+    // |v|^2 - |u|^2 - sum_k |S_k . (v - u)|.  (With c normal to the a-b
+    // plane, every offset with a c part loses to its in-plane part:
+    // hexagonal and monoclinic supercells keep 4 offsets of 12.)  A
+    // dropped offset only ties, on the wrapped cell's boundary, with the
+    // offset that beats it: the minimum distance stays the same, the image
+    // that gives it may not.
+    bool dropped = true;
+    while (dropped) {
+      dropped = false;
+      for (int a = 0; a < g.n_delta && !dropped; ++a)
+        for (int b = 0; b < g.n_delta && !dropped; ++b) {
+          if (a == b) continue;
+          const double* v = g.dS[a];
+          const double* u = g.dS[b];
+          const double w[3] = {v[0] - u[0], v[1] - u[1], v[2] - u[2]};
+          double reach = 0.0;
+          for (int k = 0; k < 3; ++k)
+            reach += std::fabs(cell[3 * k] * w[0] + cell[3 * k + 1] * w[1] +
+                               cell[3 * k + 2] * w[2]);
+          const double gain = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) -
+                              (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+          if (reach <= gain * (1.0 + 1e-12)) {
+            for (int m = a; m + 1 < g.n_delta; ++m)
+              for (int c = 0; c < 3; ++c) g.dS[m][c] = g.dS[m + 1][c];
+            --g.n_delta;
+            dropped = true;
+          }
+        }
+    }
+  }
   for (int k = g.n_delta; k < 26; ++k)
     g.dS[k][0] = g.dS[k][1] = g.dS[k][2] = 0.0;
 }
@@ -110,27 +148,56 @@ __device__ __forceinline__ double mic_r2(double dx, double dy, double dz,
   return best;
 }
 
-// Final table entry from the reciprocal accumulator `acc` (= sum over the
-// half space of w cos, i.e. (Y Y^T)[i,j]).
+// Real-space part of entry (i, j): erfc(sqrt(alpha) r) / (2 r) / eps within
+// r_cut (the diagonal without the 1/r), else 0; counts the pair in `cnt`.
+// The square root only near the cutoff and no division by a unit eps: the
+// same numbers as r = sqrt(r^2), r < r_cut and x / eps throughout.
+// (dx, dy, dz) = pos[j] - pos[i].
+__device__ __forceinline__ double real_part_d(int i, int j, double dx,
+                                              double dy, double dz,
+                                              const MicGeometry& g,
+                                              const EwaldScalars& es,
+                                              unsigned& cnt) {
+  const double r2 = mic_r2(dx, dy, dz, g);
+  double real = 0.0;
+  if (r2 < es.r_cut2) {
+    const double r = sqrt(r2);
+    if (r < es.r_cut) {
+      ++cnt;
+      double v = erfc(es.sqrt_alpha * r) / 2;
+      if (i != j) v /= r;
+      real = es.unit_eps ? v : v / es.eps;
+    }
+  }
+  return real;
+}
+
+__device__ __forceinline__ double real_part(int i, int j,
+                                            const double* __restrict__ pos,
+                                            const MicGeometry& g,
+                                            const EwaldScalars& es,
+                                            unsigned& cnt) {
+  return real_part_d(i, j, pos[3 * j] - pos[3 * i],
+                     pos[3 * j + 1] - pos[3 * i + 1],
+                     pos[3 * j + 2] - pos[3 * i + 2], g, es, cnt);
+}
+
+// Entry (i, j) from its real-space part and the reciprocal accumulator
+// `acc` (= sum over the half space of w cos, i.e. (Y Y^T)[i,j]).
+__device__ __forceinline__ double finish_entry(int i, int j,
+                                               const EwaldScalars& es,
+                                               double real, double acc) {
+  double out = real + (es.unit_eps ? 2.0 * acc : (2.0 * acc) / es.eps);
+  if (i == j) out += es.self_term;
+  return out;
+}
+
 __device__ __forceinline__ double assemble_entry(int i, int j,
                                         const double* __restrict__ pos,
                                         const MicGeometry& g,
                                         const EwaldScalars& es, double acc,
                                         unsigned& cnt) {
-  const double dx = pos[3 * j] - pos[3 * i];
-  const double dy = pos[3 * j + 1] - pos[3 * i + 1];
-  const double dz = pos[3 * j + 2] - pos[3 * i + 2];
-  const double r = sqrt(mic_r2(dx, dy, dz, g));
-  double real = 0.0;
-  if (r < es.r_cut) {
-    ++cnt;
-    double v = erfc(es.sqrt_alpha * r) / 2;
-    if (i != j) v /= r;
-    real = v / es.eps;
-  }
-  double out = real + (2.0 * acc) / es.eps;
-  if (i == j) out += es.self_term;
-  return out;
+  return finish_entry(i, j, es, real_part(i, j, pos, g, es, cnt), acc);
 }
 
 __device__ inline void count_pairs(unsigned cnt,
@@ -147,10 +214,11 @@ __device__ inline void count_pairs(unsigned cnt,
 // rows column by column (each column's rows bi <= bj of the group), so that
 // a column block of Y serves up to 8 consecutive tiles and the group's row
 // blocks stay in cache along the walk.
-__device__ inline void tile_coords_upper(int n, int tile, int& bi, int& bj) {
+// Tile of position w in that order (host and device).
+__host__ __device__ inline void tile_coords_upper_at(int n, int tile, int w,
+                                                     int& bi, int& bj) {
   const int nb = (n + tile - 1) / tile;
   const int T = nb * (nb + 1) / 2;
-  int w = blockIdx.x;
   const int full = (T / 8) * 8;
   if (w < full) {
     const int per = full / 8;
@@ -158,7 +226,7 @@ __device__ inline void tile_coords_upper(int n, int tile, int& bi, int& bj) {
   }
   constexpr int G = 8;
   for (int g0 = 0; g0 < nb; g0 += G) {
-    const int gs = min(G, nb - g0);
+    const int gs = G < nb - g0 ? G : nb - g0;
     const int head = gs * (gs + 1) / 2;  // columns g0 .. g0 + gs - 1
     const int count = head + (nb - g0 - gs) * gs;
     if (w >= count) {
@@ -180,7 +248,11 @@ __device__ inline void tile_coords_upper(int n, int tile, int& bi, int& bj) {
     }
     return;
   }
-  bi = bj = 0;  // not reached: blockIdx.x < T
+  bi = bj = 0;  // not reached: w < T
+}
+
+__device__ inline void tile_coords_upper(int n, int tile, int& bi, int& bj) {
+  tile_coords_upper_at(n, tile, int(blockIdx.x), bi, bj);
 }
 
 // Entry (i, j) of a tile and, off the diagonal block, its mirror (j, i),

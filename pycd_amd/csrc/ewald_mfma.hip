@@ -423,20 +423,185 @@ table_mfma_bf16x3_2stage(const __bf16* __restrict__ Yhi,
 // block is not stored, its upper mirror writes it (as a 128 x 128 tile off
 // the diagonal does), and the two diagonal 128-blocks store both (i, j) and
 // (j, i) as formed.  Y holds whole 256-row tiles and Kp is a multiple of
-// 2 kB, so loads and LDS stores are unconditional.
+// 2 kB, so loads and LDS stores are unconditional.  The real-space part of
+// each entry it forms is in A already (table_real_part), so the epilogue
+// adds the reciprocal and self terms to it: the same operations, in the
+// same order, as store_pair.
 constexpr int kMTile2 = 256;
+
+// Two 128 x 128 tiles in one 512-thread workgroup of the 256 x 256 kernel,
+// one per half (4 wavefronts of 64 x 64, as in the 128 x 128 kernels), each
+// half in its own 80 KB of the LDS, the two in step at the shared
+// barriers.  `sub` lists the tiles (tile row, tile column); workgroup k
+// takes entries 2k and 2k + 1 (a missing second one repeats the first and
+// stores nothing).
+__device__ __forceinline__ void bf16x3_tile_pair(
+    __bf16* S, const __bf16* __restrict__ Yhi, const __bf16* __restrict__ Ylo,
+    int n, int Kp, const EwaldScalars& es, double* __restrict__ A,
+    const int2* __restrict__ sub, int n_sub, int k) {
+  constexpr int kB = 32;
+  constexpr int kRow = kB + 8;
+  constexpr int kArr = kMTile * kRow;
+  const int t = threadIdx.x, th = t & 255, lane = t & 63;
+  const int half = __builtin_amdgcn_readfirstlane(t >> 8);
+  const int wave = __builtin_amdgcn_readfirstlane((t >> 6) & 3);
+  int idx = 2 * k + half;
+  const bool dummy = idx >= n_sub;
+  if (dummy) idx = 2 * k;
+  const int2 tl = sub[idx];
+  const int i0 = tl.x * kMTile, j0 = tl.y * kMTile;
+  const bool mirror = tl.x != tl.y;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  __bf16* Sh = S + half * (2 * 4 * kArr);
+  v4f acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  constexpr int kQ = kB / 16;
+  const int lrow = th >> 1, lk = (th & 1) * (kB / 2);
+  const int64_t aoff = int64_t(i0 + lrow) * Kp + lk;
+  const int64_t boff = int64_t(j0 + lrow) * Kp + lk;
+  struct Regs { uint4 v[4][kQ]; };
+  auto gload = [&](Regs& r, int k0) {
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      r.v[0][q] = load16_global(Yhi + aoff + k0 + 8 * q);
+      r.v[1][q] = load16_global(Ylo + aoff + k0 + 8 * q);
+      r.v[2][q] = load16_global(Yhi + boff + k0 + 8 * q);
+      r.v[3][q] = load16_global(Ylo + boff + k0 + 8 * q);
+    }
+  };
+  auto sstore = [&](int stage, const Regs& r) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int q = 0; q < kQ; ++q)
+        *reinterpret_cast<uint4*>(Sh + (4 * stage + a) * kArr + lrow * kRow +
+                                  lk + 8 * q) = r.v[a][q];
+  };
+  auto lds_barrier = [] {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  };
+  auto multiply = [&](int stage) {
+    const __bf16* Ah = Sh + (4 * stage + 0) * kArr;
+    const __bf16* Al = Sh + (4 * stage + 1) * kArr;
+    const __bf16* Bh = Sh + (4 * stage + 2) * kArr;
+    const __bf16* Bl = Sh + (4 * stage + 3) * kArr;
+    const int kof = (lane >> 4) * 8;
+    v8bf b_h[4], b_l[4];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int rb = (wc + 16 * y + (lane & 15)) * kRow + kof;
+      b_h[y] = *reinterpret_cast<const v8bf*>(Bh + rb);
+      b_l[y] = *reinterpret_cast<const v8bf*>(Bl + rb);
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int ra = (wr + 16 * x + (lane & 15)) * kRow + kof;
+      const v8bf a_h = *reinterpret_cast<const v8bf*>(Ah + ra);
+      const v8bf a_l = *reinterpret_cast<const v8bf*>(Al + ra);
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            a_l, b_h[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            a_h, b_l[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            a_h, b_h[y], acc[x][y], 0, 0, 0);
+      }
+    }
+  };
+  Regs r;
+  auto step = [&](int stage, int k0) {
+    gload(r, min(k0 + kB, Kp - kB));
+    __builtin_amdgcn_sched_barrier(0);
+    multiply(stage);
+    __builtin_amdgcn_sched_barrier(0);
+    sstore(stage ^ 1, r);
+    lds_barrier();
+  };
+  gload(r, 0);
+  sstore(0, r);
+  lds_barrier();
+  for (int k0 = 0; k0 < Kp; k0 += 2 * kB) {
+    step(0, k0);
+    step(1, k0 + kB);
+  }
+  if (dummy) return;
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r4;
+        const int j = j0 + wc + 16 * y + (lane & 15);
+        if (i < n && j < n) {
+          const double v = finish_entry(i, j, es, A[int64_t(i) * n + j],
+                                        double(acc[x][y][r4]));
+          A[int64_t(i) * n + j] = v;
+          if (mirror) A[int64_t(j) * n + i] = v;
+        }
+      }
+}
+
+// Real-space part of every entry that a pass over 128 x 128 tiles forms as
+// (i, j) (tile row <= tile column), into A ahead of the contraction, with
+// the pairs within r_cut counted as store_pair counts them (a tile off the
+// diagonal also for its mirror).  One 256-thread workgroup per tile, full
+// occupancy: the minimum image and erfc run while no matrix work waits.
+__global__ void __launch_bounds__(256)
+table_real_part(const double* __restrict__ pos, int n, MicGeometry g,
+                EwaldScalars es, double* __restrict__ A,
+                unsigned long long* __restrict__ pairs) {
+  int bi, bj;
+  tile_coords_upper(n, kMTile, bi, bj);
+  const int i0 = bi * kMTile, j0 = bj * kMTile;
+  const bool mirror = bi != bj;
+  // a lane keeps its column j; a wavefront walks rows i the same for all
+  // of its lanes (every other row, the two halves of the workgroup
+  // interleaved), so the row positions come through scalar loads
+  const int j = j0 + (threadIdx.x & (kMTile - 1));
+  const int ih = __builtin_amdgcn_readfirstlane(threadIdx.x / kMTile);
+  const bool j_ok = j < n;
+  const double pjx = j_ok ? pos[3 * j] : 0.0;
+  const double pjy = j_ok ? pos[3 * j + 1] : 0.0;
+  const double pjz = j_ok ? pos[3 * j + 2] : 0.0;
+  const int i_end = min(kMTile, n - i0);
+  unsigned cnt = 0;
+  for (int k = ih; k < i_end; k += 2) {
+    const int i = i0 + k;
+    if (j_ok) {
+      const unsigned before = cnt;
+      A[int64_t(i) * n + j] =
+          real_part_d(i, j, pjx - pos[3 * i], pjy - pos[3 * i + 1],
+                      pjz - pos[3 * i + 2], g, es, cnt);
+      if (mirror) cnt += cnt - before;
+    }
+  }
+  count_pairs(cnt, pairs);
+}
+
 __global__ void __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(2, 2)))
 table_mfma_bf16x3_256(const __bf16* __restrict__ Yhi,
                       const __bf16* __restrict__ Ylo, int n, int Kp,
-                      const double* __restrict__ pos, MicGeometry g,
                       EwaldScalars es, double* __restrict__ A,
-                      unsigned long long* __restrict__ pairs) {
+                      int main_tiles, const int2* __restrict__ sub,
+                      int n_sub) {
   constexpr int kB = 32;
   constexpr int kRow = kB + 8;  // padded row (bf16 elements)
   constexpr int kArr = kMTile2 * kRow;
   // stage s, array a (hi A, lo A, hi B, lo B) at S + (4 s + a) kArr
   __shared__ __attribute__((aligned(16))) __bf16 S[2 * 4 * kArr];
+  // the tiles past the last whole round of 256 x 256 tiles, as pairs of
+  // 128 x 128 tiles (the last round takes about half as long)
+  if (int(blockIdx.x) >= main_tiles) {
+    bf16x3_tile_pair(S, Yhi, Ylo, n, Kp, es, A, sub, n_sub,
+                     int(blockIdx.x) - main_tiles);
+    return;
+  }
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   int bi, bj;
@@ -529,7 +694,6 @@ table_mfma_bf16x3_256(const __bf16* __restrict__ Yhi,
     step(1, k0 + kB);
   }
   if (skip) return;
-  unsigned cnt = 0;
 #pragma unroll
   for (int x = 0; x < 8; ++x)
 #pragma unroll
@@ -538,11 +702,13 @@ table_mfma_bf16x3_256(const __bf16* __restrict__ Yhi,
       for (int r = 0; r < 4; ++r) {
         const int i = i0 + wr + 16 * x + (lane >> 4) * 4 + r;
         const int j = j0 + wc + 16 * y + (lane & 15);
-        if (i < n && j < n)
-          store_pair(A, n, i, j, mirror, pos, g, es, double(acc[x][y][r]),
-                     cnt);
+        if (i < n && j < n) {
+          const double v = finish_entry(i, j, es, A[int64_t(i) * n + j],
+                                        double(acc[x][y][r]));
+          A[int64_t(i) * n + j] = v;
+          if (mirror) A[int64_t(j) * n + i] = v;
+        }
       }
-  count_pairs(cnt, pairs);
 }
 
 int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
@@ -595,9 +761,44 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
   else if (bvar == 3)
     table_mfma_bf16x3_2stage<<<tiles, 256, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp,
                                                    pos, geo, es, out, pairs);
-  else if (bvar == 4)
-    table_mfma_bf16x3_256<<<tiles, 512, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, pos,
-                                                geo, es, out, pairs);
+  else if (bvar == 4) {
+    const int nb1 = (n + kMTile - 1) / kMTile;
+    table_real_part<<<nb1 * (nb1 + 1) / 2, 256, 0, s>>>(pos, n, geo, es, out,
+                                                        pairs);
+    PYCD_HIP(ctx, hipGetLastError());
+    // whole rounds of 256 x 256 tiles (one workgroup per CU), the rest as
+    // pairs of 128 x 128 tiles; fewer tiles than one round all go whole
+    int dev_cus = 0;
+    PYCD_HIP(ctx, hipDeviceGetAttribute(&dev_cus,
+                                        hipDeviceAttributeMultiprocessorCount,
+                                        ctx->device));
+    const int round = std::max(dev_cus, 1);
+    const int main_tiles = tiles >= round ? (tiles / round) * round : tiles;
+    std::vector<int2> sub;
+    for (int w = main_tiles; w < tiles; ++w) {
+      int bi = 0, bj = 0;
+      tile_coords_upper_at(n, kMTile2, w, bi, bj);
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          const int si = 2 * bi + a, sj = 2 * bj + b;
+          if (si > sj || si * kMTile >= n || sj * kMTile >= n) continue;
+          sub.push_back(make_int2(si, sj));
+        }
+    }
+    DevBuf<int2> d_sub;
+    if (!sub.empty()) {
+      PYCD_HIP(ctx, d_sub.alloc(sub.size()));
+      PYCD_HIP(ctx, hipMemcpyAsync(d_sub.ptr, sub.data(),
+                                   sizeof(int2) * sub.size(),
+                                   hipMemcpyHostToDevice, s));
+    }
+    const int grid = main_tiles + int((sub.size() + 1) / 2);
+    table_mfma_bf16x3_256<<<grid, 512, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, es,
+                                               out, main_tiles, d_sub.ptr,
+                                               int(sub.size()));
+    PYCD_HIP(ctx, hipGetLastError());
+    PYCD_HIP(ctx, hipStreamSynchronize(s));
+  }
   else if (bslice == 32)
     table_mfma_bf16x3<32><<<tiles, 256, 0, s>>>(yhi.ptr, ylo.ptr, n, Kp, pos,
                                                 geo, es, out, pairs);

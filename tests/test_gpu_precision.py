@@ -63,7 +63,8 @@ def test_precision_configuration_10x10x10():
     """10x10x10 corundum (12000 sites), |k| < 2 pi 10 / L_a: K_half = 4918
     and k_max (9, 9, 28) as the reference's half space gives them; eight
     full FP64 rows against the oracle at 1e-12 of the table's scale; the
-    FP32 and BF16x3 tables against FP64 within the bounds above; then
+    FP32 and BF16x3 tables against FP64 within the bounds above, the BF16x3
+    one equal bit for bit across its kernels; then
     20000 hops of 32 electrons on each table, with the energy bookkeeping
     equal to q^T A q of the final state on that table."""
     from oracle import ewald
@@ -79,6 +80,19 @@ def test_precision_configuration_10x10x10():
             w.system.r_cut, w.system.k_cut, 1.0, mode)
         assert reps[mode]["n_kvec"] == 4918
         assert list(reps[mode]["k_max"]) == [9, 9, 28]
+    # the default BF16x3 kernel here: 1024 256 x 256 tiles (four whole
+    # rounds of the CUs) and 104 more as 416 128 x 128 tiles in pairs; the
+    # same table bit for bit as the 128 x 128 two-stage kernel
+    try:
+        _native.set_table_variant("bf16x3", 3)
+        alt, _ = _native.pair_table(
+            pos, w.neighbors.simulation_cell_matrix,
+            w.system.translational_matrix, w.system.pbc, w.system.alpha,
+            w.system.r_cut, w.system.k_cut, 1.0, "bf16x3")
+    finally:
+        _native.set_table_variant("bf16x3", 0)
+    assert np.array_equal(alt, tabs["bf16x3"])
+    del alt
     ref = tabs["f64"]
     scale = np.max(np.abs(ref))
     rows = [0, 1, 11, 4321, 5999, 6000, 11988, 11999]
