@@ -155,6 +155,29 @@ def test_precision_configuration_10x10x10():
         assert rel < 0.05
 
 
+def test_bf16x3_tile_pairs_with_an_odd_count():
+    """Monoclinic 12 x 12 x 12 (6912 sites): on 256 CUs, 378 256 x 256
+    tiles, 256 of them whole and the other 122 as 479 128 x 128 tiles in
+    pairs, the last pair's second half repeating the first and storing
+    nothing.  The BF16x3 table equals the 128 x 128 two-stage kernel's bit
+    for bit."""
+    w = workloads.monoclinic(12)
+    pos = w.neighbors.bulk_sites.cell_coordinates
+    assert len(pos) == 6912
+    args = (pos, w.neighbors.simulation_cell_matrix,
+            w.system.translational_matrix, w.system.pbc, w.system.alpha,
+            w.system.r_cut, w.system.k_cut, 1.0)
+    tabs = []
+    try:
+        for v in (4, 3):
+            _native.set_table_variant("bf16x3", v)
+            tabs.append(_native.pair_table(*args, "bf16x3")[0])
+    finally:
+        _native.set_table_variant("bf16x3", 0)
+    assert np.array_equal(tabs[0], tabs[1])
+    assert np.all(np.isfinite(tabs[0]))
+
+
 def test_table_variants_agree_and_autotune_picks_one():
     """Each precision's tile variants (FP64: 64 x 64 and 128 x 128 tiles;
     BF16x3: K slices of 32 and 64, and of 32 in two LDS stages; 2592 sites,
