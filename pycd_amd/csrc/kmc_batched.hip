@@ -520,7 +520,6 @@ __device__ __forceinline__ void kmc_batched_body(const BatchArgs& a) {
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x;
   const int rmax = a.nc_max * (1 + a.z_max);
-  const int pmax = a.nc_max * a.z_max;
   uint32_t* mt = reinterpret_cast<uint32_t*>(smem_raw);
   double* psi = reinterpret_cast<double*>(smem_raw + 2560);
   PTab<kF32Rate>* ptab = reinterpret_cast<PTab<kF32Rate>*>(psi + rmax);
