@@ -119,8 +119,7 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
   uint32_t* pnpk = opk + nc;                              // nproc
   __shared__ double sh_t, sh_energy, sh_dg;
   __shared__ double sh_exp[64];  // 2^(j/64)
-  __shared__ int64_t sh_start, sh_end, sh_hops, sh_row_lo, sh_row_hi,
-      sh_dp_hi;
+  __shared__ int64_t sh_start, sh_hops, sh_row_lo, sh_row_hi, sh_dp_hi;
   __shared__ int32_t sh_stop, sh_noselect, sh_mover;
 
   const int tid = threadIdx.x;
@@ -325,7 +324,6 @@ kmc_plain_kernel(KmcDev d, const double* __restrict__ U,
           if (end >= d.P) end = d.P;
           sh_row_hi = end;
         }
-        sh_end = end;
         sh_hops += 1;
         // loop condition of the reference: continue while end < P
         sh_stop = (sh_row_hi > sh_row_lo && sh_row_hi >= d.P) ? 1 : 0;
@@ -1057,7 +1055,7 @@ __global__ void carrier_energy_kernel(const double* __restrict__ A,
   part[threadIdx.x] = acc;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    if (int(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = part[0];
@@ -1073,7 +1071,7 @@ __global__ void dot_kernel(const double* __restrict__ a,
   part[threadIdx.x] = acc;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    if (int(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = part[0];

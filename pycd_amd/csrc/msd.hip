@@ -47,7 +47,7 @@ msd_kernel(const double* __restrict__ pos, int64_t P, int nc, int64_t M,
   for (int l = 0; l < kLagBlock; ++l) part[l][threadIdx.x] = acc[l];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
+    if (int(threadIdx.x) < w)
 #pragma unroll
       for (int l = 0; l < kLagBlock; ++l)
         part[l][threadIdx.x] += part[l][threadIdx.x + w];
