@@ -76,7 +76,10 @@ typedef struct {
   int32_t precision;     /* PYCD_PREC_*                                     */
   double volume;         /* |T0 . (T1 x T2)|                                */
   double ms_sf;          /* structure-factor kernel time                    */
-  double ms_real;        /* fused real-space share (estimated, see DESIGN)  */
+  double ms_real;        /* 0: the real-space term is not timed apart; it   */
+                         /* is part of ms_recip (in the table kernels'      */
+                         /* epilogue, or a kernel of its own ahead of the   */
+                         /* BF16x3 256 x 256-tile contraction)              */
   double ms_recip;       /* reciprocal contraction + epilogue kernel time   */
   double ms_total;       /* device time of the whole table                  */
 } pycd_ewald_report;
