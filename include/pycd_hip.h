@@ -76,10 +76,10 @@ typedef struct {
   int32_t precision;     /* PYCD_PREC_*                                     */
   double volume;         /* |T0 . (T1 x T2)|                                */
   double ms_sf;          /* structure-factor kernel time                    */
-  double ms_real;        /* 0: the real-space term is not timed apart; it   */
-                         /* is part of ms_recip (in the table kernels'      */
-                         /* epilogue, or a kernel of its own ahead of the   */
-                         /* BF16x3 256 x 256-tile contraction)              */
+  double ms_real;        /* time of the real-space kernel ahead of the      */
+                         /* BF16x3 256 x 256-tile contraction (included in  */
+                         /* ms_recip); 0 where the term is in the table     */
+                         /* kernel's epilogue                               */
   double ms_recip;       /* reciprocal contraction + epilogue kernel time   */
   double ms_total;       /* device time of the whole table                  */
 } pycd_ewald_report;

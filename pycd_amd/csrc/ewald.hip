@@ -150,15 +150,18 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
                       const double* kv, const double* sw, int K,
                       const MicGeometry& geo, const EwaldScalars& es,
                       double* out, unsigned long long* pairs,
-                      hipEvent_t after_sf);
+                      hipEvent_t after_sf, hipEvent_t after_real,
+                      bool& real_apart);
 
 int launch_table(pycd_ctx* ctx, int precision, const double* pos, int n,
                  const double* kv, const double* sw, int K,
                  const MicGeometry& geo, const EwaldScalars& es, double* out,
-                 unsigned long long* pairs, hipEvent_t after_sf) {
+                 unsigned long long* pairs, hipEvent_t after_sf,
+                 hipEvent_t after_real, bool& real_apart) {
+  real_apart = false;
   if (precision != PYCD_PREC_F64)
     return launch_table_mfma(ctx, precision, pos, n, kv, sw, K, geo, es, out,
-                             pairs, after_sf);
+                             pairs, after_sf, after_real, real_apart);
   hipStream_t s = ctx->stream;
   const int Kp = ((2 * K + kSlice - 1) / kSlice) * kSlice + (K == 0 ? kSlice : 0);
   DevBuf<double> Y;
@@ -171,6 +174,8 @@ int launch_table(pycd_ctx* ctx, int precision, const double* pos, int n,
     PYCD_HIP(ctx, hipGetLastError());
   }
   PYCD_HIP(ctx, hipEventRecord(after_sf, s));
+  // (the real-space term is in the epilogue: no time of its own)
+  PYCD_HIP(ctx, hipEventRecord(after_real, s));
   // tile variant: 1 = 64 x 64, 2 = 128 x 128; 0 picks 128 from 2048 sites
   int v = ctx->table_variant[PYCD_PREC_F64];
   if (v == 0) v = n >= 2048 ? 2 : 1;
@@ -304,11 +309,13 @@ extern "C" int pycd_pair_table(pycd_ctx* ctx, const double* pos, int64_t n,
     out = d_A.ptr;
   }
 
-  hipEvent_t ev[3];
+  // structure factors, [separate real-space kernel], contraction
+  hipEvent_t ev[4];
   for (auto& e : ev) PYCD_HIP(ctx, hipEventCreate(&e));
   PYCD_HIP(ctx, hipEventRecord(ev[0], s));
+  bool real_apart = false;
   rc = launch_table(ctx, precision, d_pos.ptr, N, d_kv.ptr, d_sw.ptr, K, st.geo,
-                    es, out, d_pairs.ptr, ev[1]);
+                    es, out, d_pairs.ptr, ev[1], ev[3], real_apart);
   if (rc) {
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
@@ -321,9 +328,10 @@ extern "C" int pycd_pair_table(pycd_ctx* ctx, const double* pos, int64_t n,
   PYCD_HIP(ctx, hipMemcpyAsync(&pairs, d_pairs.ptr, sizeof(pairs),
                                hipMemcpyDeviceToHost, s));
   PYCD_HIP(ctx, hipStreamSynchronize(s));
-  float ms_sf = 0, ms_rest = 0;
+  float ms_sf = 0, ms_rest = 0, ms_real = 0;
   (void)hipEventElapsedTime(&ms_sf, ev[0], ev[1]);
   (void)hipEventElapsedTime(&ms_rest, ev[1], ev[2]);
+  (void)hipEventElapsedTime(&ms_real, ev[1], ev[3]);
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (report) {
     report->n_kvec = K;
@@ -332,7 +340,7 @@ extern "C" int pycd_pair_table(pycd_ctx* ctx, const double* pos, int64_t n,
     report->precision = precision;
     report->volume = st.volume;
     report->ms_sf = ms_sf;
-    report->ms_real = 0.0;
+    report->ms_real = real_apart ? ms_real : 0.0;
     report->ms_recip = ms_rest;
     report->ms_total = double(ms_sf) + ms_rest;
   }

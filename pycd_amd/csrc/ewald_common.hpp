@@ -279,6 +279,7 @@ __device__ __forceinline__ void store_pair(double* __restrict__ A, int n, int i,
 int launch_table(pycd_ctx* ctx, int precision, const double* pos, int n,
                  const double* kv, const double* sw, int K,
                  const MicGeometry& geo, const EwaldScalars& es, double* out,
-                 unsigned long long* pairs, hipEvent_t after_sf);
+                 unsigned long long* pairs, hipEvent_t after_sf,
+                 hipEvent_t after_real, bool& real_apart);
 
 }  // namespace pycd

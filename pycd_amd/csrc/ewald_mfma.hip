@@ -715,7 +715,8 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
                       const double* kv, const double* sw, int K,
                       const MicGeometry& geo, const EwaldScalars& es,
                       double* out, unsigned long long* pairs,
-                      hipEvent_t after_sf) {
+                      hipEvent_t after_sf, hipEvent_t after_real,
+                      bool& real_apart) {
   hipStream_t s = ctx->stream;
   // K padded to whole slices of the mode's kernel (zero columns add nothing)
   // BF16x3 default: 256 x 256 tiles from 4096 sites on, K slices of 64
@@ -754,6 +755,10 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
     PYCD_HIP(ctx, hipGetLastError());
   }
   PYCD_HIP(ctx, hipEventRecord(after_sf, s));
+  // the real-space term has a kernel of its own (timed) only ahead of the
+  // 256 x 256-tile contraction; elsewhere it is in the epilogue
+  real_apart = precision == PYCD_PREC_BF16X3 && bvar == 4;
+  if (!real_apart) PYCD_HIP(ctx, hipEventRecord(after_real, s));
   const int tiles = nb * (nb + 1) / 2;
   if (precision == PYCD_PREC_F32)
     table_mfma_f32<<<tiles, 256, 0, s>>>(y32.ptr, n, Kp, pos, geo, es, out,
@@ -766,6 +771,7 @@ int launch_table_mfma(pycd_ctx* ctx, int precision, const double* pos, int n,
     table_real_part<<<nb1 * (nb1 + 1) / 2, 256, 0, s>>>(pos, n, geo, es, out,
                                                         pairs);
     PYCD_HIP(ctx, hipGetLastError());
+    PYCD_HIP(ctx, hipEventRecord(after_real, s));
     // whole rounds of 256 x 256 tiles (one workgroup per CU), the rest as
     // pairs of 128 x 128 tiles; fewer tiles than one round all go whole
     int dev_cus = 0;
